@@ -1,0 +1,68 @@
+"""Bloch-Redfield oracle (TEST INFRASTRUCTURE ONLY).
+
+* ``redfield_tensor`` -- ``lime/oqs.py:528-579``: in the eigenbasis of H,
+      R = -i op2sop(diag E) - sum_k op2sop(A_k) (left(Lam_k) - right(Lam_k^dag)),
+      Lam_k = C_k o A_k, C_k[n, m] = S_k(-(E_n - E_m)) (real), Hermitian a_ops required.
+* ``rhs_hilbert``     -- the same generator applied in Hilbert space:
+      d rho/dt = -i[E, rho] - sum_k [A_k, Lam_k rho - rho Lam_k^dag]  (4K GEMMs).
+* ``propagate``       -- ``lime/oqs.py:373-468`` (``_redfield``): RK4 on row-major vec rho
+  in the eigenbasis; states are transformed back with ``v rho v^dag``; observables are
+  taken in the eigenbasis (transformed e_ops).
+
+Pinned by: the Liouville form against the Hilbert form (independent constructions);
+the population block of the secular Redfield rate matrix against golden-rule rates
+2|A_nm|^2 S(w) (closed form); trace conservation.
+"""
+import numpy as np
+from scipy.linalg import eigh
+
+from .lindblad import dag, rk4
+from .superop import left, op2sop, right
+
+
+def isherm(a, tol=1e-8):
+    a = np.asarray(a)
+    return np.allclose(a, np.conj(a).T, atol=tol)
+
+
+def bath_matrices(H, a_ops, spectra):
+    for a in a_ops:
+        if not isherm(a):
+            raise TypeError("Operators in a_ops must be Hermitian.")
+    evals, evecs = eigh(np.asarray(H))
+    W = np.real(evals[:, None] - evals[None, :])
+    A = [dag(evecs) @ np.asarray(a) @ evecs for a in a_ops]
+    C = [np.vectorize(s)(-W).astype(np.float64) for s in spectra]
+    Lam = [C[k] * A[k] for k in range(len(a_ops))]
+    return evals, evecs, A, Lam
+
+
+def redfield_tensor(H, a_ops, spectra):
+    evals, evecs, A, Lam = bath_matrices(H, a_ops, spectra)
+    R = -1j * op2sop(np.diag(evals))
+    for a, l in zip(A, Lam):
+        R = R - op2sop(a) @ (left(l) - right(dag(l)))
+    return R, evecs
+
+
+def rhs_hilbert(rho, E, A, Lam):
+    out = -1j * (E[:, None] * rho - rho * E[None, :])
+    for a, l in zip(A, Lam):
+        x = l @ rho - rho @ dag(l)
+        out = out - (a @ x - x @ a)
+    return out
+
+
+def propagate(R, evecs, rho0, dt, Nt, e_ops=()):
+    """RK4 on vec rho in the eigenbasis (oqs.py:373-468); returns (states, observables)."""
+    n = rho0.shape[0]
+    rho = (dag(evecs) @ np.asarray(rho0) @ evecs).flatten().astype(np.complex128)
+    e_eb = [dag(evecs) @ np.asarray(e) @ evecs for e in e_ops]
+    states = np.empty((Nt, n, n), dtype=np.complex128)
+    obs = np.zeros((Nt, len(e_ops)), dtype=np.complex128)
+    for k in range(Nt):
+        rho = rk4(rho, lambda v, M: M @ v, dt, R)
+        tmp = rho.reshape(n, n)
+        states[k] = evecs @ tmp @ dag(evecs)
+        obs[k] = [np.trace(e @ tmp) for e in e_eb]
+    return states, obs
