@@ -1,0 +1,102 @@
+/*
+ * lime_hip.h -- C ABI of liblime_hip.so, the MI355X (gfx950) propagator library for
+ * lime's open-quantum-system RK4 path.
+ *
+ * Conventions shared by every entry point
+ *   - All pointers are DEVICE pointers owned by the caller (e.g. torch tensors); the
+ *     library never frees them.  Matrices are dense, row-major, contiguous.
+ *   - complex64 = interleaved (re, im) float pairs (torch.complex64 layout);
+ *     complex128 = interleaved double pairs (torch.complex128 layout).
+ *   - Every call is asynchronous on `stream` (a hipStream_t; NULL = default stream).
+ *   - Return value: LIME_OK (0) or a negative LIME_ERR_* code; lime_last_error() then
+ *     returns a thread-local, human-readable message.  No C++ exception crosses the ABI.
+ *
+ * The reference (binggu56/lime) is pure Python: it has no FFI.  Each entry point below
+ * replaces the NumPy loop named in its comment; the Python facade in lime_amd/oqs.py keeps
+ * the reference's call surface and calls these through ctypes.
+ */
+#ifndef LIME_HIP_H
+#define LIME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIME_OK 0
+#define LIME_ERR_ARG (-1)          /* invalid argument (shape, null pointer, size)      */
+#define LIME_ERR_UNSUPPORTED (-2)  /* valid request this build has no kernel for        */
+#define LIME_ERR_HIP (-3)          /* a HIP runtime call failed                         */
+
+/* lime_lindblad_batch.flags */
+#define LIME_F_RENORM_TRACE 1      /* renormalise Tr rho = 1 after every step (density
+                                      matrices only; never set for QRT operators)       */
+#define LIME_F_PREC_MASK 0x30      /* arithmetic of the right-hand-side GEMMs:          */
+#define LIME_F_PREC_FP32 0x00      /*   complex64 operands on FP32 MFMA (default)       */
+#define LIME_F_PREC_3XBF16 0x10    /*   3-term BF16 split products on BF16 MFMA         */
+#define LIME_F_PREC_FP64 0x20      /*   complex128 operands on FP64 MFMA                */
+
+/*
+ * A uniform batch of `batch` propagated operators, all of Hilbert dimension n.
+ * Operator sets (H_eff, D, dense L) are shared by `group` consecutive operators:
+ * operator b uses set b / group.  Observable sets are shared by `eop_group` operators.
+ *
+ * Right-hand side (H_eff folding of every jump operator):
+ *   f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_k L_k X L_k^dag
+ *   H_eff = H - (i/2) sum_(all jump ops) L^dag L;  D = sum_(diagonal L=diag(l)) l l^dag;
+ *   L_k = the dense (non-diagonal) jump operators.
+ */
+typedef struct lime_lindblad_batch {
+  int32_t batch;       /* number of propagated operators                               */
+  int32_t n;           /* Hilbert dimension                                            */
+  int32_t group;       /* operators per operator set (>= 1)                            */
+  int32_t n_dense;     /* dense jump operators per set                                 */
+  int32_t n_eops;      /* observables per operator                                     */
+  int32_t eop_group;   /* operators per observable set (>= 1)                          */
+  int32_t flags;       /* LIME_F_*                                                     */
+  int32_t reserved;
+  const void* heff;    /* complex64  [sets][n][n]  (complex128 if PREC_FP64)           */
+  const void* dmat;    /* complex64  [sets][n][n]  or NULL (complex128 if PREC_FP64)   */
+  const void* lops;    /* complex64  [sets][n_dense][n][n] or NULL (c128 if PREC_FP64) */
+  const void* eops_t;  /* complex128 [eop_sets][n_eops][n][n], each TRANSPOSED (E^T)  */
+  void* rho;           /* complex128 [batch][n][n], state, updated in place            */
+  void* obs;           /* complex128 [batch][rec_capacity][n_eops] or NULL             */
+  void* states;        /* complex128 [batch][rec_capacity][n][n] or NULL               */
+} lime_lindblad_batch;
+
+typedef struct lime_step_params {
+  double dt;
+  int32_t nsteps;       /* RK4 steps taken by this call                                */
+  int32_t nout;         /* record after every nout-th step (global step count)         */
+  int32_t step_offset;  /* global index of this call's first step (chunked runs)       */
+  int32_t rec_capacity; /* records allocated per operator in obs / states              */
+} lime_step_params;
+
+/*
+ * Fused batched Lindblad RK4: all four stages of `nsteps` steps in one launch, the
+ * operator resident on chip, GEMMs on MFMA, state update / trace / observables in FP64.
+ * Records Tr[E rho] = sum_ij E^T_ij rho_ij after global step g when (g+1) % nout == 0,
+ * at record index (g+1)/nout - 1.
+ *
+ * Replaces the time loop of lime/oqs.py:1590-1688 (_lindblad: rk4 at phys.py:636-649
+ * over liouvillian/lindbladian at oqs.py:706-723, obs_dm at phys.py:837-844) and of the
+ * QRT drivers oqs.py:726-800 (_correlation_2p_1t), oqs.py:1227-1331 and
+ * lime/correlation.py:17-69, which all step the same right-hand side.
+ */
+int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream);
+
+/* Largest Hilbert dimension lime_lindblad_rk4 accepts for the given n_dense / flags. */
+int lime_lindblad_max_n(int32_t n_dense, int32_t flags);
+
+/* Thread-local message describing the last error returned on this thread. */
+const char* lime_last_error(void);
+
+/* Library version string. */
+const char* lime_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIME_HIP_H */

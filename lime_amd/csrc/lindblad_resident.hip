@@ -1,0 +1,425 @@
+// Fused batched Lindblad RK4, one workgroup per propagated operator, state resident on chip.
+//
+// Replaces the NumPy time loop of lime's _lindblad (lime/oqs.py:1590-1688):
+//   rho = rk4(rho, liouvillian, dt, H, c_ops)                   lime/phys.py:636-649
+//   liouvillian = -1j*comm(H, rho) + sum lindbladian(c, rho)    lime/oqs.py:706-723
+//   observables[k] = [obs_dm(rho, e) for e in e_ops]            lime/phys.py:837-844
+// with the right-hand side folded to
+//   f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_k L_k X L_k^dag.
+//
+// Work decomposition (wave64, 16x16x4 FP32 MFMA tiles):
+//   * the operator is zero-padded to NP = 16*NB; its NB*NB output tiles of 16x16 are
+//     dealt to the W waves of the workgroup, tile t = w + u*W for u < BPW;
+//   * lane l owns, in each of its tiles, the 4 elements (4*(l/16)+v, l%16), v = 0..3 --
+//     the MFMA accumulator layout -- so the FP64 state (rho and the RK4 accumulator)
+//     stays in VGPRs for the whole launch and never round-trips through memory;
+//   * LDS holds H_eff and the current stage operand S as complex64 with a row pitch of
+//     NP+1 elements (odd pitch: column-wise A-fragment reads hit distinct banks);
+//   * per stage: C = H_eff S - S H_eff^dag (+ i sum_k T_k L_k^dag, T_k = L_k S staged
+//     through LDS) on FP32 MFMA; k = -i C + D o S; the RK4 combination in FP64; the next
+//     stage operand is rounded to complex64 into LDS;
+//   * records (observables Tr[E rho], optional states) are reduced in FP64.
+// Padded rows/columns hold exact zeros in H_eff, D, L and S, so they stay zero.
+#include "common.hpp"
+#include "lindblad_resident_decl.hpp"
+
+namespace lime {
+
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC>
+__global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(MINW)))
+lindblad_rk4_resident(ResidentArgs a) {
+  constexpr int NP = NB * 16;
+  constexpr int P = NP + 1;
+  constexpr int T = W * 64;
+  constexpr int NT = NB * NB;
+  static_assert(W * BPW >= NT, "every tile needs an owner");
+
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  float2* sH = reinterpret_cast<float2*>(lds_raw);
+  float2* sS = sH + NP * P;
+  // 4 spare rows after S absorb the (unused) fragment prefetch past the last k step
+  float2* sT = sS + NP * P + 4 * P + 4;
+  double2* sRed = reinterpret_cast<double2*>(sT + (DENSE ? NP * P : 0));
+
+  const int b = blockIdx.x;
+  if (b >= a.batch) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const int n = a.n;
+  const int set = b / a.group;
+  const int nn = n * n;   // n <= 96: element offsets fit in 32 bits
+  const int lr = lane & 15;
+  const int lg = lane >> 4;
+
+  // ---- stage H_eff into LDS (zero padded to NP)
+  {
+    const float2* H = a.heff + (size_t)set * nn;
+    for (int idx = tid; idx < NP * NP; idx += T) {
+      const int i = idx / NP, j = idx - (idx / NP) * NP;
+      float2 v = make_float2(0.f, 0.f);
+      if (i < n && j < n) v = H[(size_t)i * n + j];
+      sH[i * P + j] = v;
+    }
+  }
+
+  // ---- tile ownership: first row / column of each owned tile (wave-uniform)
+  int trow[BPW], tcol[BPW];
+  bool tval[BPW];
+#pragma unroll
+  for (int u = 0; u < BPW; ++u) {
+    const int t = w + u * W;
+    tval[u] = t < NT;
+    const int tt = tval[u] ? t : NT - 1;   // idle slots shadow a real tile, never store
+    trow[u] = (tt / NB) * 16;
+    tcol[u] = (tt % NB) * 16;
+  }
+
+  double2 rho[BPW][4];   // FP64 state
+  float2 inc[BPW][4];    // RK4 increment dt*(k1 + 2k2 + 2k3 + k4)/6, summed in FP32
+  {
+    const double2* R = a.rho + (size_t)b * nn;
+#pragma unroll
+    for (int u = 0; u < BPW; ++u) {
+      const int j = tcol[u] + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = trow[u] + 4 * lg + v;
+        double2 x = make_double2(0.0, 0.0);
+        if (tval[u] && i < n && j < n) x = R[i * n + j];
+        rho[u][v] = x;
+        inc[u][v] = make_float2(0.f, 0.f);
+        if (tval[u]) sS[i * P + j] = make_float2((float)x.x, (float)x.y);
+      }
+    }
+  }
+  __syncthreads();
+
+  const double dt = a.dt;
+  const bool renorm = (a.flags & 1) != 0;
+  const bool has_d = a.dmat != nullptr;
+  const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
+
+  for (int s = 0; s < a.nsteps; ++s) {
+#pragma unroll 1
+    for (int st = 0; st < 4; ++st) {
+      // Opaque copies of the lane indices: global addresses derived from them are
+      // rebuilt inside the loop instead of being hoisted (and spilled) as 64-bit values.
+      int olr = lr, olg = lg;
+      asm volatile("" : "+v"(olr), "+v"(olg));
+      floatx4 cr[BPW], ci[BPW];
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+        cr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        ci[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+      // C += H_eff S  and  C -= S H_eff^dag  ((S H^dag)_ij = sum_k S_ik conj(H_jk)).
+      // Software pipelined by hand: fragments of step k0+4 are read from LDS while the
+      // MFMAs of step k0 issue, so at most two fragment sets are live.
+      {
+        int oR[BPW], oC[BPW];
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          oR[u] = (trow[u] + lr) * P + lg;   // row-fragment offset (A of H S, A of S H^dag)
+          oC[u] = lg * P + tcol[u] + lr;     // column fragment of S (B of H S)
+        }
+        int oHc[BPW];
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) oHc[u] = (tcol[u] + lr) * P + lg;  // B of S H^dag
+        float2 x[BPW], y[BPW], z[BPW], h[BPW];
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          x[u] = sH[oR[u]];
+          y[u] = sS[oC[u]];
+          z[u] = sS[oR[u]];
+          h[u] = sH[oHc[u]];
+        }
+#pragma unroll 2
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+          float2 xn[BPW], yn[BPW], zn[BPW], hn[BPW];
+          // prefetch step k0+4; at k0+4 == NP this reads the padding and is discarded
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            xn[u] = sH[oR[u] + k0 + 4];
+            yn[u] = sS[oC[u] + (k0 + 4) * P];
+            zn[u] = sS[oR[u] + k0 + 4];
+            hn[u] = sH[oHc[u] + k0 + 4];
+          }
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
+            ci[u] = mfma_f32(x[u].x, y[u].y, ci[u]);
+            cr[u] = mfma_f32(-x[u].y, y[u].y, cr[u]);
+            ci[u] = mfma_f32(x[u].y, y[u].x, ci[u]);
+            cr[u] = mfma_f32(-z[u].x, h[u].x, cr[u]);
+            ci[u] = mfma_f32(z[u].x, h[u].y, ci[u]);
+            cr[u] = mfma_f32(-z[u].y, h[u].y, cr[u]);
+            ci[u] = mfma_f32(-z[u].y, h[u].x, ci[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            x[u] = xn[u];
+            y[u] = yn[u];
+            z[u] = zn[u];
+            h[u] = hn[u];
+          }
+        }
+      }
+      // C += i sum_k (L_k S) L_k^dag
+      for (int kk = 0; DENSE && kk < a.n_dense; ++kk) {
+        const float2* L = a.lops + ((size_t)set * a.n_dense + kk) * nn;
+        floatx4 tr[BPW], ti[BPW];
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          tr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+          ti[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+          const int kc = k0 + lg;
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            const int ra = trow[u] + lr;
+            float2 x = make_float2(0.f, 0.f);
+            if (ra < n && kc < n) x = L[trow[u] * n + olr * n + k0 + olg];
+            const float2 y = sS[kc * P + tcol[u] + lr];
+            tr[u] = mfma_f32(x.x, y.x, tr[u]);
+            tr[u] = mfma_f32(-x.y, y.y, tr[u]);
+            ti[u] = mfma_f32(x.x, y.y, ti[u]);
+            ti[u] = mfma_f32(x.y, y.x, ti[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          if (!tval[u]) continue;
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            sT[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = make_float2(tr[u][v], ti[u][v]);
+        }
+        __syncthreads();
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+          const int kc = k0 + lg;
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            const float2 x = sT[(trow[u] + lr) * P + kc];
+            const int jr = tcol[u] + lr;
+            float2 h = make_float2(0.f, 0.f);
+            if (jr < n && kc < n) h = L[tcol[u] * n + olr * n + k0 + olg];
+            cr[u] = mfma_f32(x.x, h.y, cr[u]);
+            cr[u] = mfma_f32(-x.y, h.x, cr[u]);
+            ci[u] = mfma_f32(x.x, h.x, ci[u]);
+            ci[u] = mfma_f32(x.y, h.y, ci[u]);
+          }
+        }
+        __syncthreads();
+      }
+      // k = -i C + D o S, formed in place: afterwards kr = ci, ki = -cr.
+      if (has_d) {
+        const float2* Dm = a.dmat + (size_t)set * nn;
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          if (!tval[u]) continue;
+          const int j = tcol[u] + lr;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = trow[u] + 4 * lg + v;
+            float2 d = make_float2(0.f, 0.f);
+            if (i < n && j < n) d = Dm[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+            const float2 x = sS[i * P + j];
+            ci[u][v] += d.x * x.x - d.y * x.y;
+            cr[u][v] -= d.x * x.y + d.y * x.x;
+            __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
+          }
+        }
+      }
+      __syncthreads();  // every wave has finished reading S
+
+      const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
+      const double cnext = (st == 2) ? dt : 0.5 * dt;
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float kr = ci[u][v], ki = -cr[u][v];
+          float2 ic = (st == 0) ? make_float2(0.f, 0.f) : inc[u][v];
+          ic.x = fmaf(cacc, kr, ic.x);
+          ic.y = fmaf(cacc, ki, ic.y);
+          if (st < 3) {
+            inc[u][v] = ic;
+            if (tval[u]) {
+              const double sx = rho[u][v].x + cnext * (double)kr;
+              const double sy = rho[u][v].y + cnext * (double)ki;
+              sS[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = make_float2((float)sx, (float)sy);
+            }
+          } else {
+            rho[u][v].x += (double)ic.x;   // the state update itself is FP64
+            rho[u][v].y += (double)ic.y;
+          }
+        }
+      }
+      if (st < 3) {
+        __syncthreads();  // S of the next stage is complete
+        continue;
+      }
+
+      // ---------------- end of step: optional trace renormalisation, records
+      if (renorm) {
+        double tr = 0.0;
+#pragma unroll
+        for (int u = 0; u < BPW; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (tval[u] && trow[u] == tcol[u] && 4 * lg + v == lr) tr += rho[u][v].x;
+        tr = wave_sum(tr);
+        if (lane == 0) sRed[w].x = tr;
+        __syncthreads();
+        double tot = 0.0;
+        for (int x = 0; x < W; ++x) tot += sRed[x].x;
+        const double inv = 1.0 / tot;
+#pragma unroll
+        for (int u = 0; u < BPW; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            rho[u][v].x *= inv;
+            rho[u][v].y *= inv;
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+        if (!tval[u]) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          sS[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] =
+              make_float2((float)rho[u][v].x, (float)rho[u][v].y);
+      }
+
+      const int g = a.step_offset + s + 1;
+      const bool rec_now = record_any && (g % a.nout) == 0 && (g / a.nout - 1) < a.rec_capacity;
+      if (rec_now) {
+        const int rec = g / a.nout - 1;
+        if (a.obs != nullptr) {
+          const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
+          for (int e = 0; e < a.n_eops; ++e) {
+            const double2* E = E0 + (size_t)e * nn;
+            double pr = 0.0, pi = 0.0;
+#pragma unroll
+            for (int u = 0; u < BPW; ++u) {
+              const int j = tcol[u] + lr;
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int i = trow[u] + 4 * lg + v;
+                if (tval[u] && i < n && j < n) {
+                  const double2 ev = E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+                  pr += ev.x * rho[u][v].x - ev.y * rho[u][v].y;
+                  pi += ev.x * rho[u][v].y + ev.y * rho[u][v].x;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+              }
+            }
+            pr = wave_sum(pr);
+            pi = wave_sum(pi);
+            // slots W.. are free: slots 0..W-1 may still be read by the trace step
+            if (lane == 0) sRed[W + e * W + w] = make_double2(pr, pi);
+          }
+        }
+        if (a.states != nullptr) {
+          double2* Sg = a.states + ((size_t)b * a.rec_capacity + rec) * nn;
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            const int j = tcol[u] + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int i = trow[u] + 4 * lg + v;
+              if (tval[u] && i < n && j < n) Sg[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr] = rho[u][v];
+            }
+          }
+        }
+      }
+      __syncthreads();  // S of the next step (and the record partials) are complete
+      if (rec_now && a.obs != nullptr && tid < a.n_eops) {
+        const int rec = g / a.nout - 1;
+        double2 t = make_double2(0.0, 0.0);
+        for (int x = 0; x < W; ++x) {
+          const double2 y = sRed[W + tid * W + x];
+          t.x += y.x;
+          t.y += y.y;
+        }
+        a.obs[((size_t)b * a.rec_capacity + rec) * a.n_eops + tid] = t;
+      }
+    }
+  }
+
+  // ---- write the state back
+  {
+    double2* R = a.rho + (size_t)b * nn;
+#pragma unroll
+    for (int u = 0; u < BPW; ++u) {
+      const int j = tcol[u] + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = trow[u] + 4 * lg + v;
+        if (tval[u] && i < n && j < n) R[i * n + j] = rho[u][v];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------ host launch
+namespace {
+
+// One launch shape per padded size NB: W waves, BPW tiles per wave (W * BPW >= NB^2),
+// chosen so that the FP64 state (16 doubles per tile per lane) fits the VGPR budget of
+// the occupancy the LDS footprint allows.
+constexpr int kWaves[7] = {0, 1, 2, 3, 8, 13, 12};
+
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC>
+hipError_t launch_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+  auto k = lindblad_rk4_resident<NB, W, BPW, MINW, DENSE, REC>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(a.batch), dim3(W * 64), lds, stream, a);
+  return hipGetLastError();
+}
+
+template <int NB, int W, int BPW, int MINW, bool DENSE>
+hipError_t launch_rec(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+  const bool rec = a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
+  return rec ? launch_one<NB, W, BPW, MINW, DENSE, true>(a, lds, stream)
+             : launch_one<NB, W, BPW, MINW, DENSE, false>(a, lds, stream);
+}
+
+template <int NB, int W, int BPW, int MINW>
+hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+  if constexpr (NB <= 5) {
+    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true>(a, lds, stream);
+  } else {
+    if (a.n_dense > 0) return hipErrorInvalidValue;
+  }
+  return launch_rec<NB, W, BPW, MINW, false>(a, lds, stream);
+}
+
+}  // namespace
+
+int resident_nb_max(int n_dense) { return n_dense > 0 ? 5 : 6; }
+
+size_t resident_lds_bytes(int nb, int n_dense, int n_eops) {
+  const int np = nb * 16, p = np + 1;
+  size_t bytes = ((size_t)(n_dense > 0 ? 3 : 2) * np * p + 4 * p + 4) * sizeof(float2);
+  bytes += (size_t)(1 + n_eops) * kWaves[nb] * sizeof(double2);
+  return bytes;
+}
+
+hipError_t resident_launch(const ResidentArgs& a, int nb, hipStream_t stream) {
+  const size_t lds = resident_lds_bytes(nb, a.n_dense, a.n_eops);
+  switch (nb) {
+    case 1: return launch_shape<1, 1, 1, 4>(a, lds, stream);
+    case 2: return launch_shape<2, 2, 2, 4>(a, lds, stream);
+    case 3: return launch_shape<3, 3, 3, 3>(a, lds, stream);
+    case 4: return launch_shape<4, 8, 2, 4>(a, lds, stream);
+    case 5: return launch_shape<5, 13, 2, 4>(a, lds, stream);
+    case 6: return launch_shape<6, 12, 3, 3>(a, lds, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace lime

@@ -1,0 +1,26 @@
+// Host-visible interface of the resident Lindblad RK4 kernel (lindblad_resident.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace lime {
+
+struct ResidentArgs {
+  int batch, n, group, n_dense, n_eops, eop_group, flags;
+  const float2* heff;
+  const float2* dmat;
+  const float2* lops;
+  const double2* eops_t;
+  double2* rho;
+  double2* obs;
+  double2* states;
+  double dt;
+  int nsteps, nout, step_offset, rec_capacity;
+};
+
+int resident_nb_max(int n_dense);
+size_t resident_lds_bytes(int nb, int n_dense, int n_eops);
+hipError_t resident_launch(const ResidentArgs& a, int nb, hipStream_t stream);
+
+}  // namespace lime

@@ -1,0 +1,189 @@
+"""Batched Lindblad RK4 propagation on the GPU through liblime_hip.so.
+
+``LindbladPropagator`` owns the device-side operator sets of a batch (H_eff, the
+Hadamard dissipator D, the dense jump operators) and advances complex128 operators
+``rho`` (density matrices or quantum-regression operators) with the fused kernel
+``lime_lindblad_rk4``.  It is the engine behind the lime-compatible facade in
+``lime_amd.oqs``; it replaces the per-step NumPy loop of lime's ``_lindblad``
+(lime/oqs.py:1590-1688) and of the QRT drivers (oqs.py:726-800, 1227-1331).
+
+Right-hand side (all jump operators folded):
+    f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_dense L X L^dag,
+    H_eff = H - (i/2) sum_all L^dag L,  D = sum_{diagonal L = diag(l)} l l^dag.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_PREC_FLAGS = {"fp32": _lib.LIME_F_PREC_FP32, "3xbf16": _lib.LIME_F_PREC_3XBF16,
+               "fp64": _lib.LIME_F_PREC_FP64}
+
+
+def device():
+    """The GPU the library runs on; raises when none is visible (no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("lime_amd needs an AMD Instinct GPU (ROCm): no GPU is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def as_c128(x, dev):
+    """Array-like or tensor -> contiguous complex128 tensor on ``dev``."""
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=torch.complex128)
+    else:
+        if hasattr(x, "toarray"):          # scipy.sparse
+            x = x.toarray()
+        t = torch.as_tensor(np.asarray(x, dtype=np.complex128), device=dev)
+    return t.contiguous()
+
+
+def _stack_sets(x, nsets, n, dev, what):
+    t = as_c128(x, dev)
+    if t.dim() == 2:
+        t = t.unsqueeze(0)
+    if t.shape[-2:] != (n, n):
+        raise ValueError(f"{what} has shape {tuple(t.shape)}, expected (..., {n}, {n})")
+    if t.shape[0] == 1 and nsets > 1:
+        t = t.expand(nsets, n, n)
+    if t.shape[0] != nsets:
+        raise ValueError(f"{what} has {t.shape[0]} sets, expected {nsets}")
+    return t.contiguous()
+
+
+def fold_jump_operators(H, c_ops, dev=None):
+    """Split jump operators into H_eff, the Hadamard dissipator D and dense operators.
+
+    H: (S, N, N) or (N, N); c_ops: list of (S, N, N) or (N, N) operators (shared).
+    Returns complex128 device tensors heff (S,N,N), dmat (S,N,N) or None,
+    lops (S,Kd,N,N) or None.  A jump operator is *diagonal* when every off-diagonal
+    entry of every set is exactly zero.
+    """
+    dev = dev or device()
+    Ht = as_c128(H, dev)
+    if Ht.dim() == 2:
+        Ht = Ht.unsqueeze(0)
+    if Ht.dim() != 3 or Ht.shape[-1] != Ht.shape[-2]:
+        raise ValueError(f"H must be (N, N) or (S, N, N), got {tuple(Ht.shape)}")
+    S, n = Ht.shape[0], Ht.shape[-1]
+    gamma = torch.zeros_like(Ht)
+    dmat = None
+    dense = []
+    for k, c in enumerate(c_ops or []):
+        L = _stack_sets(c, S, n, dev, f"c_ops[{k}]")
+        gamma = gamma + torch.matmul(L.conj().transpose(-1, -2), L)
+        diag = torch.diagonal(L, dim1=-2, dim2=-1)
+        off = L - torch.diag_embed(diag)
+        if not bool(torch.any(off != 0)):
+            term = diag.unsqueeze(-1) * diag.conj().unsqueeze(-2)
+            dmat = term if dmat is None else dmat + term
+        else:
+            dense.append(L)
+    heff = (Ht - 0.5j * gamma).contiguous()
+    lops = torch.stack(dense, dim=1).contiguous() if dense else None
+    return heff, (dmat.contiguous() if dmat is not None else None), lops
+
+
+class LindbladPropagator:
+    """Operator sets for a batch and the kernel call that advances it.
+
+    Parameters
+    ----------
+    H : (N, N) or (S, N, N) array-like / tensor
+    c_ops : list of (N, N) or (S, N, N) jump operators (may be empty/None)
+    group : propagated operators per operator set (operator b uses set b // group)
+    precision : 'fp32' (FP32 MFMA, default), '3xbf16', 'fp64'
+    """
+
+    def __init__(self, H, c_ops=None, group=1, precision="fp32"):
+        self.dev = device()
+        if precision not in _PREC_FLAGS:
+            raise ValueError(f"unknown precision {precision!r}; expected one of {sorted(_PREC_FLAGS)}")
+        self.precision = precision
+        heff, dmat, lops = fold_jump_operators(H, c_ops, self.dev)
+        self.nsets, self.n = heff.shape[0], heff.shape[-1]
+        self.group = int(group)
+        if self.group < 1:
+            raise ValueError("group must be >= 1")
+        cdt = torch.complex128 if precision == "fp64" else torch.complex64
+        self.heff = heff.to(cdt).contiguous()
+        self.dmat = dmat.to(cdt).contiguous() if dmat is not None else None
+        self.lops = lops.to(cdt).contiguous() if lops is not None else None
+        self.n_dense = 0 if lops is None else lops.shape[1]
+        self._lib = _lib.load()
+
+    @property
+    def batch(self):
+        return self.nsets * self.group
+
+    def max_n(self):
+        return self._lib.lime_lindblad_max_n(self.n_dense, _PREC_FLAGS[self.precision])
+
+    def step(self, rho, dt, nsteps, *, eops_t=None, eop_group=None, obs=None, states=None,
+             nout=1, step_offset=0, rec_capacity=0, renorm=False, stream=None):
+        """Advance ``rho`` (B, N, N) complex128 in place by ``nsteps`` RK4 steps (async)."""
+        n, B = self.n, rho.shape[0]
+        if rho.dtype != torch.complex128 or not rho.is_contiguous() or rho.device != self.dev:
+            raise ValueError("rho must be a contiguous complex128 tensor on the GPU")
+        if rho.shape != (B, n, n) or B != self.batch:
+            raise ValueError(f"rho has shape {tuple(rho.shape)}, expected ({self.batch}, {n}, {n})")
+        d = _lib.LindbladBatch()
+        d.batch, d.n, d.group, d.n_dense = B, n, self.group, self.n_dense
+        d.n_eops = 0 if eops_t is None else int(eops_t.shape[-3])
+        d.eop_group = int(eop_group or B)
+        d.flags = _PREC_FLAGS[self.precision] | (_lib.LIME_F_RENORM_TRACE if renorm else 0)
+        d.heff = self.heff.data_ptr()
+        d.dmat = self.dmat.data_ptr() if self.dmat is not None else None
+        d.lops = self.lops.data_ptr() if self.lops is not None else None
+        d.eops_t = eops_t.data_ptr() if eops_t is not None else None
+        d.rho = rho.data_ptr()
+        d.obs = obs.data_ptr() if obs is not None else None
+        d.states = states.data_ptr() if states is not None else None
+        p = _lib.StepParams()
+        p.dt, p.nsteps, p.nout = float(dt), int(nsteps), int(nout)
+        p.step_offset, p.rec_capacity = int(step_offset), int(rec_capacity)
+        s = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        _lib.check(self._lib.lime_lindblad_rk4(ctypes.byref(d), ctypes.byref(p),
+                                               ctypes.c_void_p(s.cuda_stream)))
+
+    def run(self, rho0, dt, nsteps, e_ops=None, nout=1, store_states=False, renorm=False,
+            eop_group=None, chunk=None):
+        """Propagate ``rho0`` for ``nsteps`` steps, recording after every ``nout`` steps.
+
+        rho0 : (N, N) (broadcast to the batch) or (B, N, N)
+        e_ops : list of (N, N) shared observables, or a tensor (E_sets, n_e, N, N) with
+                observable set b // eop_group for operator b.
+        Returns (rho (B,N,N), obs (B, Nt//nout, n_e) or None, states (B, Nt//nout, N, N) or None),
+        all complex128 device tensors.  Records sit at t_r = (r + 1) * nout * dt.
+        """
+        n, B = self.n, self.batch
+        rho = as_c128(rho0, self.dev)
+        if rho.dim() == 2:
+            rho = rho.unsqueeze(0).expand(B, n, n)
+        rho = rho.clone().contiguous()           # the caller's rho0 is never mutated
+        eops_t = None
+        if e_ops is not None and len(e_ops) > 0:
+            if isinstance(e_ops, torch.Tensor) and e_ops.dim() == 4:
+                E = as_c128(e_ops, self.dev)
+            else:
+                E = torch.stack([as_c128(e, self.dev) for e in e_ops]).unsqueeze(0)
+            eops_t = E.transpose(-1, -2).contiguous()
+            if eop_group is None:
+                eop_group = B // E.shape[0]
+        nrec = nsteps // nout if nout > 0 else 0
+        obs = None
+        if eops_t is not None and nrec > 0:
+            obs = torch.zeros((B, nrec, eops_t.shape[1]), dtype=torch.complex128, device=self.dev)
+        states = None
+        if store_states and nrec > 0:
+            states = torch.empty((B, nrec, n, n), dtype=torch.complex128, device=self.dev)
+        chunk = int(chunk or nsteps)
+        done = 0
+        while done < nsteps:
+            m = min(chunk, nsteps - done)
+            self.step(rho, dt, m, eops_t=eops_t, eop_group=eop_group, obs=obs, states=states,
+                      nout=nout, step_offset=done, rec_capacity=nrec, renorm=renorm)
+            done += m
+        return rho, obs, states

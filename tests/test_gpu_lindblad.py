@@ -1,0 +1,174 @@
+"""GPU parity of the fused Lindblad RK4 kernel against the FP64 oracle.
+
+Tolerances (max absolute error vs the FP64 oracle) are stated per test.  The kernel
+evaluates the right-hand side with complex64 operands on FP32 MFMA (FP32 accumulate)
+and keeps the state and its update in FP64, so errors grow like
+eps32 * dt * ||L|| * Nt * |rho| at worst.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lindblad as ol
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _prop():
+    from lime_amd.propagator import LindbladPropagator
+    return LindbladPropagator
+
+
+def rand_c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def rand_herm(rng, n, scale=1.0):
+    x = rand_c(rng, n, n)
+    h = (x + x.conj().T) / 2
+    return h * (scale / np.max(np.abs(np.linalg.eigvalsh(h))))
+
+
+def rand_dm(rng, n):
+    x = rand_c(rng, n, n)
+    r = x @ x.conj().T
+    return r / np.trace(r)
+
+
+def chain(n, eps, J):
+    H = np.diag(eps).astype(np.complex128)
+    for i in range(n - 1):
+        H[i, i + 1] = H[i + 1, i] = J
+    return H
+
+
+def test_chain_populations_config1():
+    """8-site chain, on-site dephasing, rho0 = |0><0|, dt 0.05 x 2000, populations every 10."""
+    rng = np.random.default_rng(2024)
+    n, dt, nt, nout = 8, 0.05, 2000, 10
+    H = chain(n, rng.normal(0, 0.05, n), -0.1)
+    c_ops = [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = np.zeros((n, n), complex)
+    rho0[0, 0] = 1
+    P = [np.diag(np.eye(n)[i]).astype(complex) for i in range(n)]
+    ref_rho, ref_obs, _ = ol.propagate(H, rho0, c_ops, dt, nt, P, nout=nout)
+    prop = _prop()(H, c_ops)
+    rho, obs, _ = prop.run(rho0, dt, nt, e_ops=P, nout=nout)
+    obs = obs[0].cpu().numpy()
+    err = np.max(np.abs(obs - ref_obs))
+    print(f"config1 max population error {err:.3e}")
+    assert obs.shape == (nt // nout, n)
+    assert err < 1e-6                       # gate: <= 1e-6 max population error
+    assert np.max(np.abs(rho[0].cpu().numpy() - ref_rho)) < 1e-6
+
+
+@pytest.mark.parametrize("n", [3, 8, 16, 17, 31, 48, 64, 65, 80, 96])
+def test_random_dense_hamiltonian_dephasing(n):
+    """General Hermitian H, on-site + collective diagonal jump operators, random rho."""
+    rng = np.random.default_rng(n)
+    H = rand_herm(rng, n)
+    c_ops = [np.sqrt(0.05) * np.diag(np.eye(n)[i]) for i in range(n)]
+    c_ops.append(0.1 * np.diag(rand_c(rng, n)))        # collective diagonal operator
+    rho0 = rand_dm(rng, n)
+    dt, nt = 0.02, 100
+    ref, _, _ = ol.propagate(H, rho0, c_ops, dt, nt)
+    rho, _, _ = _prop()(H, c_ops).run(rho0, dt, nt)
+    err = np.max(np.abs(rho[0].cpu().numpy() - ref))
+    print(f"n={n} max element error {err:.3e}")
+    assert err < 2e-7 * max(1.0, 1.0)
+
+
+@pytest.mark.parametrize("n", [4, 10, 16, 33, 64, 80])
+def test_dense_jump_operators(n):
+    rng = np.random.default_rng(100 + n)
+    H = rand_herm(rng, n)
+    c_ops = [0.1 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(3)]
+    c_ops += [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = rand_dm(rng, n)
+    dt, nt = 0.02, 60
+    ref, _, _ = ol.propagate(H, rho0, c_ops, dt, nt)
+    prop = _prop()(H, c_ops)
+    assert prop.n_dense == 3
+    rho, _, _ = prop.run(rho0, dt, nt)
+    err = np.max(np.abs(rho[0].cpu().numpy() - ref))
+    print(f"n={n} dense-L max element error {err:.3e}")
+    assert err < 5e-7
+
+
+def test_cavity_golden_vectors_on_gpu(golden_dir):
+    """lime's examples/cor.dat and dm.dat reproduced by the GPU kernel (dense a, a^dag)."""
+    g = np.load(os.path.join(golden_dir, "cavity_qrt.npz"))
+    n, kappa, n_th = 10, 0.25, 2.0
+    a = np.diag(np.sqrt(np.arange(1, n)), 1).astype(np.complex128)
+    num = np.diag(np.arange(n)).astype(np.complex128)
+    H = np.diag(np.arange(n) * 1.0).astype(np.complex128)
+    c_ops = [np.sqrt(kappa * (1 + n_th)) * a, np.sqrt(kappa * n_th) * a.conj().T]
+    w = np.exp(-np.arange(n) / n_th)
+    rho0 = np.diag(w / w.sum()).astype(np.complex128)
+    dt = 10.0 / 199
+    x0 = a @ rho0 @ a.conj().T
+    rho, obs, states = _prop()(H, c_ops).run(x0, dt, 200, e_ops=[num], store_states=True)
+    cor = obs[0, :, 0].cpu().numpy()
+    dm = states[0].cpu().numpy()
+    e1 = np.max(np.abs(cor - g["cor"]))
+    e2 = np.max(np.abs(dm - g["dm"]))
+    print(f"cavity golden: cor err {e1:.3e}, dm err {e2:.3e}")
+    assert e1 < 2e-6 and e2 < 5e-7
+
+
+def test_qrt_non_hermitian_operator_batch_and_groups():
+    """Batch of 2 operator sets x 3 QRT operators each (group=3), shared observable set."""
+    rng = np.random.default_rng(7)
+    n, S, G = 12, 2, 3
+    Hs = np.stack([rand_herm(rng, n) for _ in range(S)])
+    L = [np.sqrt(0.03) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_c(rng, n, n) for _ in range(S * G)])
+    A = rand_c(rng, n, n)
+    dt, nt = 0.05, 80
+    prop = _prop()(Hs, L, group=G)
+    rho, obs, _ = prop.run(X0, dt, nt, e_ops=[A], nout=4)
+    obs = obs.cpu().numpy()
+    for b in range(S * G):
+        _, ref_obs, _ = ol.propagate(Hs[b // G], X0[b], L, dt, nt, [A], nout=4)
+        # |Tr[A X]| ~ 20 here: 1e-6 relative to the largest recorded value
+        assert np.max(np.abs(obs[b] - ref_obs)) < 1e-6 * np.max(np.abs(ref_obs))
+
+
+def test_trace_renormalisation_keeps_unit_trace():
+    rng = np.random.default_rng(9)
+    n = 20
+    H = rand_herm(rng, n)
+    c_ops = [0.2 * rand_c(rng, n, n) / np.sqrt(n)] + [np.sqrt(0.05) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = rand_dm(rng, n)
+    rho, _, _ = _prop()(H, c_ops).run(rho0, 0.05, 500, renorm=True)
+    r = rho[0].cpu().numpy()
+    # Re Tr rho is renormalised to 1 every step; Im Tr rho is roundoff of the FP32 products
+    assert abs(np.trace(r).real - 1) < 1e-14
+    assert abs(np.trace(r).imag) < 1e-7
+    assert np.max(np.abs(r - r.conj().T)) < 1e-6
+    ref, _, _ = ol.propagate(H, rho0, c_ops, 0.05, 500)
+    assert np.max(np.abs(r - ref)) < 1e-6
+
+
+def test_chunked_run_equals_single_launch():
+    rng = np.random.default_rng(11)
+    n = 24
+    H = rand_herm(rng, n)
+    c_ops = [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = rand_dm(rng, n)
+    P = [np.diag(np.eye(n)[i]).astype(complex) for i in range(3)]
+    prop = _prop()(H, c_ops)
+    r1, o1, s1 = prop.run(rho0, 0.05, 120, e_ops=P, nout=6, store_states=True)
+    r2, o2, s2 = prop.run(rho0, 0.05, 120, e_ops=P, nout=6, store_states=True, chunk=25)
+    assert torch.equal(r1, r2) and torch.equal(o1, o2) and torch.equal(s1, s2)
+
+
+def test_unsupported_size_raises():
+    from lime_amd import _lib
+    n = 200
+    H = np.eye(n, dtype=complex)
+    prop = _prop()(H, [])
+    with pytest.raises(_lib.LimeUnsupported):
+        prop.run(np.eye(n, dtype=complex) / n, 0.01, 2)
