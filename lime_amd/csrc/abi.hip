@@ -42,7 +42,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   if (d->heff == nullptr || d->rho == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: heff and rho are required");
   if (d->n_dense < 0 || (d->n_dense > 0 && d->lops == nullptr))
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n_dense > 0 needs lops");
-  if (d->n_eops < 0 || d->n_eops > 64) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n_eops must be in [0, 64]");
+  if (d->n_eops < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n_eops < 0");
   if (d->obs != nullptr && (d->n_eops == 0 || d->eops_t == nullptr))
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: obs needs n_eops > 0 and eops_t");
   if ((d->obs != nullptr || d->states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))

