@@ -1,0 +1,206 @@
+"""lime.oqs call surface on the MI355X propagator (drop-in for lime/oqs.py's Lindblad path).
+
+Every propagation below runs the fused HIP kernel ``lime_lindblad_rk4`` through
+``lime_amd.propagator`` -- there is no NumPy time loop and no CPU fallback.
+
+Reference functions and the decisions taken where lime's behaviour is broken:
+
+* ``_lindblad``            -- oqs.py:1590-1688.  Copies rho0 (never mutated), complex128
+  state, records observables/states *after* each step (t_k = t0 + (k+1) dt).  lime's
+  ``Result`` crash (mol.py:92) is fixed (``lime_amd.mol.Result``).  With
+  ``return_result=False`` lime writes observables taken *before* the step under the time
+  *after* it (oqs.py:1638-1658); here ``obs.dat`` holds the after-step values.
+* ``_correlation_2p_1t``   -- oqs.py:726-800: <A(t)B> with rho <- B rho0, cor[k] at
+  t = (k+1) dt, the text file ``output`` in lime's ``'{t} {complex} \\n'`` format (written
+  after the run rather than per step); an unknown ``method`` raises ValueError instead of
+  lime's ``sys.exit``.
+* ``Lindblad_solver``      -- oqs.py:1117-1331 (setH / set_c_ops / set_e_ops / configure /
+  liouvillian / evolve / evolve_fast / correlation_2op_1t / correlation_3op_1t /
+  correlation_4op_1t (the overriding definition, :1301-1315) / correlation_3op_2t /
+  correlation_4op_2t).  ``correlation_3op_2t`` runs the Nt x Ntau two-time grid as ONE
+  batched launch of Nt operators instead of lime's Nt serial propagations.
+* ``liouvillian``/``lindbladian`` -- oqs.py:706-723, single right-hand-side evaluations.
+
+Extensions (lime has no batching): ``H`` may be (B, N, N) and ``rho0`` (B, N, N); keyword
+``nout`` records every nout-th step; ``precision`` selects the GEMM arithmetic;
+``renorm`` renormalises Tr rho = 1 after each step (density matrices only).
+"""
+import numpy as np
+import torch
+
+from . import phys
+from . import superoperator as superop
+from .mol import Result
+from .propagator import LindbladPropagator, as_c128, device
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _is_batched(H, rho0):
+    return np.ndim(H) == 3 or (hasattr(rho0, "ndim") and rho0.ndim == 3)
+
+
+def _run(H, rho0, c_ops, dt, Nt, e_ops, nout=1, store_states=False, renorm=False,
+         precision="fp32"):
+    """Propagate one operator or a batch; returns device tensors (rho, obs, states)."""
+    dev = device()
+    Ht = as_c128(H, dev)
+    r0 = as_c128(rho0, dev)
+    if Ht.dim() == 2 and r0.dim() == 3:               # shared H, batch of initial operators
+        group = r0.shape[0]
+    else:
+        group = 1
+    prop = LindbladPropagator(Ht, c_ops or [], group=group, precision=precision)
+    if r0.dim() == 2 and prop.batch > 1:
+        r0 = r0.unsqueeze(0).expand(prop.batch, *r0.shape).contiguous()
+    return prop.run(r0, dt, Nt, e_ops=e_ops, nout=nout, store_states=store_states, renorm=renorm)
+
+
+# ------------------------------------------------------------------ right-hand side
+def lindbladian(l, rho):
+    """L rho L^dag - 1/2 {L^dag L, rho} (oqs.py:716-723)."""
+    ld = phys.dag(l)
+    return l @ (rho @ ld) - 0.5 * phys.anticomm(ld @ l, rho)
+
+
+def liouvillian(rho, H, c_ops):
+    """-i[H, rho] + sum_k D[L_k] rho (oqs.py:706-713)."""
+    rhs = -1j * phys.comm(H, rho)
+    for c in c_ops or []:
+        rhs = rhs + lindbladian(c, rho)
+    return rhs
+
+
+# ------------------------------------------------------------------ drivers
+def _lindblad(H, rho0, c_ops, e_ops=None, Nt=1, dt=0.005, return_result=True, t0=0.0,
+              nout=1, store_states=True, renorm=False, precision="fp32"):
+    e_ops = [] if e_ops is None else list(e_ops)
+    batched = _is_batched(H, rho0)
+    rho, obs, states = _run(H, rho0, c_ops, dt, Nt, e_ops, nout=nout,
+                            store_states=(store_states and return_result), renorm=renorm,
+                            precision=precision)
+    nrec = Nt // nout
+    obs_np = _np(obs) if obs is not None else np.zeros((rho.shape[0], nrec, 0), np.complex128)
+    if not return_result:
+        with open("obs.dat", "w") as f:
+            fmt = "{} " * (len(e_ops) + 1) + "\n"
+            for r in range(nrec):
+                vals = obs_np[0, r] if not batched else obs_np[:, r].ravel()
+                f.write(fmt.format(t0 + (r + 1) * nout * dt, *vals) if not batched
+                        else " ".join(str(x) for x in (t0 + (r + 1) * nout * dt, *vals)) + "\n")
+        out = _np(rho)
+        return out if batched else out[0]
+    result = Result(dt=dt, Nt=Nt, rho0=rho0, t0=t0, nout=nout)
+    if batched:
+        result.observables = obs_np
+        result.rholist = _np(states) if states is not None else None
+    else:
+        result.observables = obs_np[0]
+        result.rholist = list(_np(states[0])) if states is not None else None
+    return result
+
+
+def _correlation_2p_1t(H, rho0, ops, c_ops, dt, Nt, method="lindblad", output="cor.dat"):
+    if method != "lindblad":
+        raise ValueError("The method {} has not been implemented yet! Please try lindblad.".format(method))
+    A, B = ops
+    x0 = as_c128(B, device()) @ as_c128(rho0, device())
+    _, obs, _ = _run(H, x0, c_ops, dt, Nt, [A])
+    cor = _np(obs)[0, :, 0] if obs.shape[0] == 1 else _np(obs)[:, :, 0]
+    if output:
+        with open(output, "w") as f:
+            for k in range(Nt):
+                f.write("{} {} \n".format((k + 1) * dt, cor[k] if cor.ndim == 1 else cor[:, k]))
+    return cor
+
+
+def _fast_lindblad(H, rho0, c_ops, e_ops=None, Nt=1, dt=0.005):
+    """lime's intended fast path (oqs.py:1485-1564): final rho only."""
+    rho, _, _ = _run(H, rho0, c_ops, dt, Nt, None)
+    out = _np(rho)
+    return out if _is_batched(H, rho0) else out[0]
+
+
+class Lindblad_solver:
+    """oqs.py:1117-1331 on the GPU kernel."""
+
+    def __init__(self, H=None, c_ops=None, e_ops=None, precision="fp32"):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+        self.H = H
+        self.precision = precision
+
+    def set_c_ops(self, c_ops):
+        self.c_ops = c_ops
+
+    def set_e_ops(self, e_ops):
+        self.e_ops = e_ops
+
+    def setH(self, H):
+        self.H = H
+
+    def configure(self, c_ops, e_ops):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+
+    def liouvillian(self):
+        return superop.liouvillian(self.H, self.c_ops)
+
+    def steady_state(self):
+        raise NotImplementedError("steady_state is a stub in lime (oqs.py:1149) as well")
+
+    def evolve(self, rho0, dt, Nt, t0=0.0, e_ops=None, return_result=True, nout=1,
+               store_states=True, renorm=False):
+        if isinstance(self.H, list):
+            from .driven import _lindblad_driven
+            return _lindblad_driven(self.H, rho0=rho0, c_ops=self.c_ops, e_ops=e_ops, Nt=Nt,
+                                    dt=dt, t0=t0)
+        return _lindblad(self.H, rho0, c_ops=self.c_ops, e_ops=e_ops, Nt=Nt, dt=dt,
+                         return_result=return_result, t0=t0, nout=nout,
+                         store_states=store_states, renorm=renorm, precision=self.precision)
+
+    def evolve_fast(self, rho0, dt, Nt, t0=0.0, e_ops=None, return_result=True, fast=True):
+        return _fast_lindblad(self.H, rho0, self.c_ops, e_ops=e_ops, Nt=Nt, dt=dt)
+
+    def correlation_2op_1t(self, rho0, a_op, b_op, dt, Nt, output="cor.dat"):
+        return _correlation_2p_1t(self.H, rho0, ops=[a_op, b_op], c_ops=self.c_ops, dt=dt,
+                                  Nt=Nt, output=output)
+
+    def correlation_3op_1t(self, rho0, oplist, dt=0.005, Nt=1):
+        """<A B(t) C>: rho <- C rho0 A, observe B (oqs.py:1227-1246)."""
+        a_op, b_op, c_op = oplist
+        dev = device()
+        x0 = as_c128(c_op, dev) @ as_c128(rho0, dev) @ as_c128(a_op, dev)
+        _, obs, _ = _run(self.H, x0, self.c_ops, dt, Nt, [b_op], precision=self.precision)
+        o = _np(obs)[:, :, 0]
+        return o[0] if o.shape[0] == 1 else o
+
+    def correlation_4op_1t(self, rho0, ops, dt=0.005, nt=1):
+        """<A B(t) C(t) D> (the overriding definition, oqs.py:1301-1315)."""
+        if len(ops) != 4:
+            raise ValueError("Number of operators is not 4.")
+        a, b, c, d = ops
+        return self.correlation_3op_1t(rho0, [a, b @ c, d], dt, nt)
+
+    def correlation_3op_2t(self, rho0, ops, dt, Nt, Ntau):
+        """<A(t) B(t+tau) C(t)> (oqs.py:1268-1299) as (Nt, Ntau) complex128.
+
+        rho(t_k) for k = 1..Nt from one propagation, then all Nt operators C rho(t_k) A
+        advance together as one batched launch over Ntau steps observing B.
+        """
+        dev = device()
+        a_op, b_op, c_op = [as_c128(o, dev) for o in ops]
+        _, _, states = _run(self.H, rho0, self.c_ops, dt, Nt, None, store_states=True,
+                            precision=self.precision)
+        X = torch.matmul(torch.matmul(c_op, states[0]), a_op).contiguous()     # (Nt, N, N)
+        _, obs, _ = _run(self.H, X, self.c_ops, dt, Ntau, [b_op], precision=self.precision)
+        return _np(obs)[:, :, 0]
+
+    def correlation_4op_2t(self, rho0, ops, dt, nt, ntau):
+        """<A(t) B(t+tau) C(t+tau) D(t)> (oqs.py:1317-1331)."""
+        if len(ops) != 4:
+            raise ValueError("Number of operators is not 4.")
+        a, b, c, d = ops
+        return self.correlation_3op_2t(rho0, [a, b @ c, d], dt, nt, ntau)

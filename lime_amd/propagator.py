@@ -108,6 +108,7 @@ class LindbladPropagator:
         if self.group < 1:
             raise ValueError("group must be >= 1")
         cdt = torch.complex128 if precision == "fp64" else torch.complex64
+        self.heff_c128 = heff
         self.heff = heff.to(cdt).contiguous()
         self.dmat = dmat.to(cdt).contiguous() if dmat is not None else None
         self.lops = lops.to(cdt).contiguous() if lops is not None else None

@@ -187,3 +187,28 @@ def correlation_3op_2t(H, rho0, ops, c_ops, dt, Nt, Ntau):
 def rk4_stability_factor(z):
     """R(z) = 1 + z + z^2/2 + z^3/6 + z^4/24: one classic RK4 step of y' = lambda y."""
     return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+def propagate_driven(H, rho0, c_ops, dt, Nt, e_ops=None, t0=0.0):
+    """lime's _lindblad_driven (oqs.py:1691-1800) without its H0 aliasing.
+
+    H = [H0, [H1, f1], ...]; H(t) = H0 - sum_i f_i(t) H_i evaluated at t = t0 + (k+1) dt
+    and held over the four RK4 stages of step k.  Returns (rho, obs (Nt, n_e), states).
+    """
+    H0 = np.asarray(H[0], dtype=np.complex128)
+    drives = [(np.asarray(h, dtype=np.complex128), f) for h, f in H[1:]]
+    e_ops = [] if e_ops is None else [np.asarray(e) for e in e_ops]
+    rho = np.array(rho0, dtype=np.complex128, copy=True)
+    c_ops = [np.asarray(c) for c in (c_ops or [])]
+    obs = np.zeros((Nt, len(e_ops)), dtype=np.complex128)
+    states = []
+    t = t0
+    for k in range(Nt):
+        t += dt
+        Ht = H0.copy()
+        for h, f in drives:
+            Ht = Ht - f(t) * h
+        rho = rk4(rho, liouvillian_ref, dt, Ht, c_ops)
+        obs[k] = [obs_dm(rho, e) for e in e_ops]
+        states.append(rho.copy())
+    return rho, obs, states

@@ -1,0 +1,174 @@
+"""The lime-compatible facade (lime_amd.oqs / superoperator / driven) on the GPU, against
+the FP64 oracle and lime's golden vectors.  Tolerances: 1e-6 relative to the largest
+value compared unless stated (FP32-MFMA right-hand side, FP64 state)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lindblad as ol
+from oracle import superop as osu
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def rand_c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def rand_herm(rng, n):
+    x = rand_c(rng, n, n)
+    h = (x + x.conj().T) / 2
+    return h / np.max(np.abs(np.linalg.eigvalsh(h)))
+
+
+def rand_dm(rng, n):
+    x = rand_c(rng, n, n)
+    r = x @ x.conj().T
+    return r / np.trace(r)
+
+
+def close(a, b, rel=1e-6):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) <= rel * max(1.0, np.max(np.abs(b)))
+
+
+def model(seed, n=6):
+    rng = np.random.default_rng(seed)
+    H = rand_herm(rng, n)
+    c_ops = [0.2 * rand_c(rng, n, n) / np.sqrt(n)] + [np.sqrt(0.05) * np.diag(np.eye(n)[i]) for i in range(n)]
+    return rng, H, c_ops
+
+
+def test_evolve_result_observables_states_times():
+    from lime_amd.oqs import Lindblad_solver
+    rng, H, c_ops = model(1)
+    rho0 = rand_dm(rng, 6)
+    P = [np.diag(np.eye(6)[i]).astype(complex) for i in range(6)]
+    s = Lindblad_solver(H, c_ops)
+    r = s.evolve(rho0, dt=0.02, Nt=50, e_ops=P)
+    _, ref_obs, ref_states = ol.propagate(H, rho0, c_ops, 0.02, 50, P, store_states=True)
+    assert r.observables.shape == (50, 6) and len(r.rholist) == 50
+    assert close(r.observables, ref_obs) and close(np.array(r.rholist), np.array(ref_states))
+    assert np.allclose(r.times, 0.02 * np.arange(1, 51))
+    assert np.array_equal(rho0, rho0.copy())          # caller's rho0 untouched
+
+
+def test_evolve_return_result_false_writes_obs_dat(tmp_path, monkeypatch):
+    from lime_amd.oqs import Lindblad_solver
+    monkeypatch.chdir(tmp_path)
+    rng, H, c_ops = model(2)
+    rho0 = rand_dm(rng, 6)
+    A = rand_herm(rng, 6)
+    rho = Lindblad_solver(H, c_ops).evolve(rho0, 0.05, 20, e_ops=[A], return_result=False)
+    ref, ref_obs, _ = ol.propagate(H, rho0, c_ops, 0.05, 20, [A])
+    assert close(rho, ref)
+    lines = open("obs.dat").read().strip().splitlines()
+    assert len(lines) == 20
+    t, v = lines[-1].split()
+    assert abs(float(t) - 1.0) < 1e-12 and abs(complex(v) - ref_obs[-1, 0]) < 1e-6
+
+
+def test_correlation_2op_1t_and_cor_dat(tmp_path, monkeypatch):
+    from lime_amd.oqs import Lindblad_solver
+    monkeypatch.chdir(tmp_path)
+    rng, H, c_ops = model(3)
+    rho0 = rand_dm(rng, 6)
+    A, B = rand_c(rng, 6, 6), rand_c(rng, 6, 6)
+    cor = Lindblad_solver(H, c_ops).correlation_2op_1t(rho0, A, B, 0.05, 40, output="cor.dat")
+    ref = ol.correlation_2op_1t(H, rho0, A, B, c_ops, 0.05, 40)
+    assert close(cor, ref)
+    rows = [l.split() for l in open("cor.dat").read().strip().splitlines()]
+    assert len(rows) == 40 and abs(float(rows[0][0]) - 0.05) < 1e-12
+    assert abs(complex(rows[5][1]) - ref[5]) < 1e-6 * np.max(np.abs(ref))
+    from lime_amd.oqs import _correlation_2p_1t
+    with pytest.raises(ValueError):
+        _correlation_2p_1t(H, rho0, [A, B], c_ops, 0.05, 4, method="heom", output=None)
+
+
+def test_cavity_golden_through_facade(golden_dir):
+    """lime's examples/cor.dat via Lindblad_solver.correlation_3op_1t([a^dag, n, a])."""
+    from lime_amd.oqs import Lindblad_solver
+    g = np.load(os.path.join(golden_dir, "cavity_qrt.npz"))
+    n, kappa, n_th = 10, 0.25, 2.0
+    a = np.diag(np.sqrt(np.arange(1, n)), 1).astype(np.complex128)
+    num = np.diag(np.arange(n)).astype(np.complex128)
+    H = np.diag(np.arange(n) * 1.0).astype(np.complex128)
+    c_ops = [np.sqrt(kappa * (1 + n_th)) * a, np.sqrt(kappa * n_th) * a.conj().T]
+    w = np.exp(-np.arange(n) / n_th)
+    rho0 = np.diag(w / w.sum()).astype(np.complex128)
+    cor = Lindblad_solver(H, c_ops).correlation_3op_1t(rho0, [a.conj().T, num, a], dt=10 / 199, Nt=200)
+    assert np.max(np.abs(cor - g["cor"])) < 2e-6
+
+
+def test_correlation_4op_1t_and_two_time():
+    from lime_amd.oqs import Lindblad_solver
+    rng, H, c_ops = model(4, n=5)
+    rho0 = rand_dm(rng, 5)
+    A, B, C, D = (rand_c(rng, 5, 5) for _ in range(4))
+    s = Lindblad_solver(H, c_ops)
+    c4 = s.correlation_4op_1t(rho0, [A, B, C, D], 0.05, 30)
+    assert close(c4, ol.correlation_3op_1t(H, rho0, A, B @ C, D, c_ops, 0.05, 30))
+    c2t = s.correlation_3op_2t(rho0, [A, B, C], 0.05, 12, 17)
+    ref = ol.correlation_3op_2t(H, rho0, [A, B, C], c_ops, 0.05, 12, 17)
+    assert c2t.shape == (12, 17) and close(c2t, ref)
+    c42 = s.correlation_4op_2t(rho0, [A, B, C, D], 0.05, 6, 9)
+    assert close(c42, ol.correlation_3op_2t(H, rho0, [A, B @ C, D], c_ops, 0.05, 6, 9))
+    with pytest.raises(ValueError):
+        s.correlation_4op_2t(rho0, [A, B, C], 0.05, 2, 2)
+
+
+def test_batched_hamiltonians():
+    from lime_amd.oqs import _lindblad
+    rng = np.random.default_rng(5)
+    n, Bn = 7, 5
+    Hs = np.stack([rand_herm(rng, n) for _ in range(Bn)])
+    c_ops = [np.sqrt(0.03) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = rand_dm(rng, n)
+    P0 = np.diag(np.eye(n)[0]).astype(complex)
+    r = _lindblad(Hs, rho0, c_ops, e_ops=[P0], Nt=40, dt=0.05)
+    assert r.observables.shape == (Bn, 40, 1)
+    for b in range(Bn):
+        _, ref, _ = ol.propagate(Hs[b], rho0, c_ops, 0.05, 40, [P0])
+        assert close(r.observables[b], ref)
+
+
+def test_driven_matches_oracle_and_keeps_h0():
+    from lime_amd.oqs import Lindblad_solver
+    rng, H0, c_ops = model(6)
+    H1 = rand_herm(rng, 6)
+    f = lambda t: 0.4 * np.sin(2.0 * t)
+    H0_copy = H0.copy()
+    rho0 = rand_dm(rng, 6)
+    P = [np.diag(np.eye(6)[i]).astype(complex) for i in range(6)]
+    r = Lindblad_solver([H0, [H1, f]], c_ops).evolve(rho0, 0.05, 40, e_ops=P)
+    _, ref, _ = ol.propagate_driven([H0, [H1, f]], rho0, c_ops, 0.05, 40, P)
+    assert close(r.observables, ref)
+    assert np.array_equal(H0, H0_copy)
+
+
+def test_eseries_solver_on_gpu():
+    from lime_amd import superoperator as so
+    rng, H, c_ops = model(7, n=4)
+    rho0 = rand_dm(rng, 4)
+    A, B, C = rand_c(rng, 4, 4), rand_c(rng, 4, 4), rand_c(rng, 4, 4)
+    t = np.linspace(0, 3, 31)
+    es = so.Lindblad_solver(H, c_ops)
+    es.eigenstates()
+    ref = osu.Eseries(H, c_ops)
+    assert np.allclose(es.correlation_2op_1t(rho0, [A, B], t), ref.correlation_2op_1t(rho0, [A, B], t), atol=1e-9)
+    assert np.allclose(es.correlation_3op_1t(rho0, [A, B, C], t), ref.correlation_3op_1t(rho0, [A, B, C], t), atol=1e-9)
+    w = np.linspace(-2, 2, 9)
+    assert np.allclose(es.correlation_2op_1w(rho0, [A, B], w), ref.correlation_2op_1w(rho0, [A, B], w), atol=1e-8)
+    assert np.allclose(es.evolve(rho0, t, [A]).observables, ref.evolve(rho0, t, [A]), atol=1e-9)
+    tau = np.linspace(0, 2, 7)
+    assert np.allclose(es.correlation_3op_2t(rho0, [A, B, C], t[:5], tau),
+                       ref.correlation_3op_2t(rho0, [A, B, C], t[:5], tau), atol=1e-9)
+
+
+def test_liouvillian_dense_on_gpu_matches_kron():
+    from lime_amd import superoperator as so
+    rng, H, c_ops = model(8, n=5)
+    Ld = so.liouvillian_dense(H, c_ops).cpu().numpy()
+    assert np.allclose(Ld, osu.liouvillian(H, c_ops), atol=1e-13)

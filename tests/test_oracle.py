@@ -308,3 +308,29 @@ def test_dephasing_lorentzian_spectrum():
     expect = ol.rk4_stability_factor(lam * dt) ** np.arange(1, nt + 1)
     assert np.max(np.abs(cor - expect)) < 1e-13
     assert np.max(np.abs(cor - np.exp(lam * t))) < 1e-6
+
+
+def test_driven_samples_field_at_step_end():
+    """Diagonal H0 and drive, no dissipation: each element picks up exactly
+    R(-i (w_ij - f(t_{k+1}) d_ij) dt) per step -- pins the t_{k+1} sampling and the sign."""
+    rng = np.random.default_rng(13)
+    n, dt, nt = 4, 0.05, 60
+    E = rng.standard_normal(n)
+    d = rng.standard_normal(n)
+    f = lambda t: 0.7 * np.cos(1.3 * t)
+    rho0 = rand_dm(rng, n)
+    H = [np.diag(E), [np.diag(d), f]]
+    H0_copy = H[0].copy()
+    rho, _, _ = ol.propagate_driven(H, rho0, [], dt, nt)
+    expect = rho0.copy()
+    for k in range(nt):
+        t = (k + 1) * dt
+        e = E - f(t) * d
+        expect = expect * ol.rk4_stability_factor(-1j * (e[:, None] - e[None, :]) * dt)
+    assert np.max(np.abs(rho - expect)) < 1e-14
+    assert np.array_equal(H[0], H0_copy)                 # H0 is not mutated
+    # a constant drive reduces to the time-independent propagation of H0 - c H1
+    c_ops = [0.2 * rand_c(rng, n, n)]
+    r1, _, _ = ol.propagate_driven([np.diag(E), [np.diag(d), lambda t: 0.3]], rho0, c_ops, dt, 20)
+    r2, _, _ = ol.propagate(np.diag(E) - 0.3 * np.diag(d), rho0, c_ops, dt, 20)
+    assert np.max(np.abs(r1 - r2)) < 1e-13
