@@ -36,6 +36,8 @@ extern "C" {
 #define LIME_F_PREC_FP32 0x00      /*   complex64 operands on FP32 MFMA (default)       */
 #define LIME_F_PREC_3XBF16 0x10    /*   3-term BF16 split products on BF16 MFMA         */
 #define LIME_F_PREC_FP64 0x20      /*   complex128 operands on FP64 MFMA                */
+#define LIME_F_FORCE_TILED 0x100   /* use the per-stage tiled kernels even when n fits the
+                                      resident kernel (testing / tuning)                  */
 
 /*
  * A uniform batch of `batch` propagated operators, all of Hilbert dimension n.
@@ -43,9 +45,14 @@ extern "C" {
  * operator b uses set b / group.  Observable sets are shared by `eop_group` operators.
  *
  * Right-hand side (H_eff folding of every jump operator):
- *   f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_k L_k X L_k^dag
+ *   f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_k L_k X R_k^dag
  *   H_eff = H - (i/2) sum_(all jump ops) L^dag L;  D = sum_(diagonal L=diag(l)) l l^dag;
- *   L_k = the dense (non-diagonal) jump operators.
+ *   L_k = the dense (non-diagonal) jump operators; R_k = L_k unless `rops` is given
+ *   (general sandwich terms, e.g. the Hilbert-space form of the Redfield tensor).
+ *
+ * n <= 96 runs the resident kernel (no workspace).  Larger n runs one launch per RK4
+ * stage and needs `work` of lime_lindblad_workspace_bytes() bytes (caller-owned device
+ * memory; the library never allocates).
  */
 typedef struct lime_lindblad_batch {
   int32_t batch;       /* number of propagated operators                               */
@@ -63,6 +70,9 @@ typedef struct lime_lindblad_batch {
   void* rho;           /* complex128 [batch][n][n], state, updated in place            */
   void* obs;           /* complex128 [batch][rec_capacity][n_eops] or NULL             */
   void* states;        /* complex128 [batch][rec_capacity][n][n] or NULL               */
+  const void* rops;    /* complex64  [sets][n_dense][n][n] right factors, or NULL (= lops) */
+  void* work;          /* device workspace for n > 96 (see lime_lindblad_workspace_bytes) */
+  int64_t work_bytes;
 } lime_lindblad_batch;
 
 typedef struct lime_step_params {
@@ -86,8 +96,11 @@ typedef struct lime_step_params {
  */
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream);
 
-/* Largest Hilbert dimension lime_lindblad_rk4 accepts for the given n_dense / flags. */
+/* Largest Hilbert dimension the resident (workspace-free) kernel accepts. */
 int lime_lindblad_max_n(int32_t n_dense, int32_t flags);
+
+/* Workspace bytes lime_lindblad_rk4 needs for this descriptor (0 when none). */
+int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d);
 
 /* Thread-local message describing the last error returned on this thread. */
 const char* lime_last_error(void);

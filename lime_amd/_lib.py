@@ -19,11 +19,13 @@ LIME_F_RENORM_TRACE = 1
 LIME_F_PREC_FP32 = 0x00
 LIME_F_PREC_3XBF16 = 0x10
 LIME_F_PREC_FP64 = 0x20
+LIME_F_FORCE_TILED = 0x100
 
 #: every symbol include/lime_hip.h declares (checked by tests/test_abi.py)
 EXPORTS = (
     "lime_lindblad_rk4",
     "lime_lindblad_max_n",
+    "lime_lindblad_workspace_bytes",
     "lime_last_error",
     "lime_version",
 )
@@ -58,6 +60,9 @@ class LindbladBatch(ctypes.Structure):
         ("rho", ctypes.c_void_p),
         ("obs", ctypes.c_void_p),
         ("states", ctypes.c_void_p),
+        ("rops", ctypes.c_void_p),
+        ("work", ctypes.c_void_p),
+        ("work_bytes", ctypes.c_int64),
     ]
 
 
@@ -91,6 +96,8 @@ def load(path=LIB_PATH):
         lib.lime_lindblad_rk4.restype = ctypes.c_int
         lib.lime_lindblad_max_n.argtypes = [ctypes.c_int32, ctypes.c_int32]
         lib.lime_lindblad_max_n.restype = ctypes.c_int
+        lib.lime_lindblad_workspace_bytes.argtypes = [ctypes.POINTER(LindbladBatch)]
+        lib.lime_lindblad_workspace_bytes.restype = ctypes.c_int64
         lib.lime_last_error.argtypes = []
         lib.lime_last_error.restype = ctypes.c_char_p
         lib.lime_version.argtypes = []

@@ -4,8 +4,8 @@
 
 #include "common.hpp"
 #include "lime_hip.h"
-
 #include "lindblad_resident_decl.hpp"
+#include "lindblad_tiled_decl.hpp"
 
 namespace {
 
@@ -20,17 +20,33 @@ int hip_fail(hipError_t e, const char* where) {
   return fail(LIME_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+constexpr size_t kLdsLimit = 160 * 1024;
+
+// Does this descriptor run on the resident (single-workgroup-per-operator) kernel?
+bool use_resident(const lime_lindblad_batch* d) {
+  if (d->flags & LIME_F_FORCE_TILED) return false;
+  const int nb = lime::cdiv(d->n, 16);
+  if (nb > lime::resident_nb_max(d->n_dense)) return false;
+  return lime::resident_lds_bytes(nb, d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* lime_last_error(void) { return g_last_error.c_str(); }
 
-const char* lime_version(void) { return "lime_hip 0.1 (gfx950)"; }
+const char* lime_version(void) { return "lime_hip 0.2 (gfx950)"; }
 
 int lime_lindblad_max_n(int32_t n_dense, int32_t flags) {
   if ((flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32) return 0;
   return 16 * lime::resident_nb_max(n_dense);
+}
+
+int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
+  if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
+  if (use_resident(d)) return 0;
+  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense);
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
@@ -50,19 +66,43 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   if (p->step_offset < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: step_offset < 0");
   if ((d->flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32)
     return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: only the FP32-MFMA precision is built");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
 
-  const int nb = lime::cdiv(d->n, 16);
-  if (nb > lime::resident_nb_max(d->n_dense))
-    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: n = " + std::to_string(d->n) +
-                                          " exceeds the resident kernel's limit " +
-                                          std::to_string(16 * lime::resident_nb_max(d->n_dense)));
-  const size_t lds = lime::resident_lds_bytes(nb, d->n_dense, d->obs ? d->n_eops : 0);
-  if (lds > 160 * 1024)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: n = " + std::to_string(d->n) + " with " +
-                                          std::to_string(d->n_dense) + " dense operators and " +
-                                          std::to_string(d->n_eops) + " observables needs " +
-                                          std::to_string(lds) + " B of LDS (> 160 KiB)");
-  lime::ResidentArgs a;
+  if (use_resident(d)) {
+    lime::ResidentArgs a;
+    a.batch = d->batch;
+    a.n = d->n;
+    a.group = d->group;
+    a.n_dense = d->n_dense;
+    a.n_eops = d->obs ? d->n_eops : 0;
+    a.eop_group = d->eop_group;
+    a.flags = d->flags;
+    a.heff = static_cast<const float2*>(d->heff);
+    a.dmat = static_cast<const float2*>(d->dmat);
+    a.lops = static_cast<const float2*>(d->lops);
+    a.rops = static_cast<const float2*>(d->rops);
+    a.eops_t = static_cast<const double2*>(d->eops_t);
+    a.rho = static_cast<double2*>(d->rho);
+    a.obs = static_cast<double2*>(d->obs);
+    a.states = static_cast<double2*>(d->states);
+    a.dt = p->dt;
+    a.nsteps = p->nsteps;
+    a.nout = (d->obs || d->states) ? p->nout : 0;
+    a.step_offset = p->step_offset;
+    a.rec_capacity = p->rec_capacity;
+    hipError_t e = lime::resident_launch(a, lime::cdiv(d->n, 16), s);
+    if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (resident) launch");
+    return LIME_OK;
+  }
+
+  // ---- tiled path: one launch per RK4 stage, workspace supplied by the caller
+  if (d->flags & LIME_F_RENORM_TRACE)
+    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: trace renormalisation is built for n <= 96 only");
+  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense);
+  if (d->work == nullptr || d->work_bytes < (int64_t)need)
+    return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n = " + std::to_string(d->n) + " needs a workspace of " +
+                                  std::to_string(need) + " bytes (lime_lindblad_workspace_bytes)");
+  lime::TiledArgs a = {};
   a.batch = d->batch;
   a.n = d->n;
   a.group = d->group;
@@ -70,20 +110,20 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   a.n_eops = d->obs ? d->n_eops : 0;
   a.eop_group = d->eop_group;
   a.flags = d->flags;
+  a.rec_capacity = p->rec_capacity;
+  a.dt = p->dt;
   a.heff = static_cast<const float2*>(d->heff);
   a.dmat = static_cast<const float2*>(d->dmat);
   a.lops = static_cast<const float2*>(d->lops);
+  a.rops = static_cast<const float2*>(d->rops);
   a.eops_t = static_cast<const double2*>(d->eops_t);
   a.rho = static_cast<double2*>(d->rho);
   a.obs = static_cast<double2*>(d->obs);
   a.states = static_cast<double2*>(d->states);
-  a.dt = p->dt;
-  a.nsteps = p->nsteps;
-  a.nout = (d->obs || d->states) ? p->nout : 0;
-  a.step_offset = p->step_offset;
-  a.rec_capacity = p->rec_capacity;
-  hipError_t e = lime::resident_launch(a, nb, static_cast<hipStream_t>(stream));
-  if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 launch");
+  const int nout = (d->obs || d->states) ? p->nout : 0;
+  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s,
+                                 lime::tiled_pick(d->batch, d->n));
+  if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (tiled) launch");
   return LIME_OK;
 }
 

@@ -168,6 +168,7 @@ lindblad_rk4_resident(ResidentArgs a) {
       // C += i sum_k (L_k S) L_k^dag
       for (int kk = 0; DENSE && kk < a.n_dense; ++kk) {
         const float2* L = a.lops + ((size_t)set * a.n_dense + kk) * nn;
+        const float2* R = (a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + kk) * nn;
         floatx4 tr[BPW], ti[BPW];
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
@@ -203,7 +204,7 @@ lindblad_rk4_resident(ResidentArgs a) {
             const float2 x = sT[(trow[u] + lr) * P + kc];
             const int jr = tcol[u] + lr;
             float2 h = make_float2(0.f, 0.f);
-            if (jr < n && kc < n) h = L[tcol[u] * n + olr * n + k0 + olg];
+            if (jr < n && kc < n) h = R[tcol[u] * n + olr * n + k0 + olg];
             cr[u] = mfma_f32(x.x, h.y, cr[u]);
             cr[u] = mfma_f32(-x.y, h.x, cr[u]);
             ci[u] = mfma_f32(x.x, h.x, ci[u]);

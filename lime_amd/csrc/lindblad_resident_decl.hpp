@@ -11,6 +11,7 @@ struct ResidentArgs {
   const float2* heff;
   const float2* dmat;
   const float2* lops;
+  const float2* rops;
   const double2* eops_t;
   double2* rho;
   double2* obs;

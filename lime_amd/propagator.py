@@ -97,8 +97,15 @@ class LindbladPropagator:
     precision : 'fp32' (FP32 MFMA, default), '3xbf16', 'fp64'
     """
 
-    def __init__(self, H, c_ops=None, group=1, precision="fp32"):
+    def __init__(self, H, c_ops=None, group=1, precision="fp32", sandwich=None,
+                 force_tiled=False):
+        """``sandwich``: optional explicit (L, R) lists of (S,N,N) operators contributing
+        sum_k L_k X R_k^dag (used instead of folding dense jump operators; H must then
+        already be the effective generator).  ``force_tiled`` selects the per-stage tiled
+        kernels even for n <= 96."""
         self.dev = device()
+        self.force_tiled = bool(force_tiled)
+        self._work = None
         if precision not in _PREC_FLAGS:
             raise ValueError(f"unknown precision {precision!r}; expected one of {sorted(_PREC_FLAGS)}")
         self.precision = precision
@@ -112,7 +119,19 @@ class LindbladPropagator:
         self.heff = heff.to(cdt).contiguous()
         self.dmat = dmat.to(cdt).contiguous() if dmat is not None else None
         self.lops = lops.to(cdt).contiguous() if lops is not None else None
-        self.n_dense = 0 if lops is None else lops.shape[1]
+        self.rops = None
+        if sandwich is not None:
+            Ls, Rs = sandwich
+            if len(Ls) != len(Rs):
+                raise ValueError("sandwich needs as many left as right factors")
+            if self.lops is not None:
+                raise ValueError("sandwich terms cannot be combined with dense jump operators")
+            if len(Ls):
+                self.lops = torch.stack([_stack_sets(x, self.nsets, self.n, self.dev, "L")
+                                         for x in Ls], dim=1).to(cdt).contiguous()
+                self.rops = torch.stack([_stack_sets(x, self.nsets, self.n, self.dev, "R")
+                                         for x in Rs], dim=1).to(cdt).contiguous()
+        self.n_dense = 0 if self.lops is None else self.lops.shape[1]
         self._lib = _lib.load()
 
     @property
@@ -134,7 +153,8 @@ class LindbladPropagator:
         d.batch, d.n, d.group, d.n_dense = B, n, self.group, self.n_dense
         d.n_eops = 0 if eops_t is None else int(eops_t.shape[-3])
         d.eop_group = int(eop_group or B)
-        d.flags = _PREC_FLAGS[self.precision] | (_lib.LIME_F_RENORM_TRACE if renorm else 0)
+        d.flags = _PREC_FLAGS[self.precision] | (_lib.LIME_F_RENORM_TRACE if renorm else 0) \
+            | (_lib.LIME_F_FORCE_TILED if self.force_tiled else 0)
         d.heff = self.heff.data_ptr()
         d.dmat = self.dmat.data_ptr() if self.dmat is not None else None
         d.lops = self.lops.data_ptr() if self.lops is not None else None
@@ -142,6 +162,13 @@ class LindbladPropagator:
         d.rho = rho.data_ptr()
         d.obs = obs.data_ptr() if obs is not None else None
         d.states = states.data_ptr() if states is not None else None
+        d.rops = self.rops.data_ptr() if self.rops is not None else None
+        need = int(self._lib.lime_lindblad_workspace_bytes(ctypes.byref(d)))
+        if need > 0:
+            if self._work is None or self._work.numel() < need:
+                self._work = torch.empty(need, dtype=torch.uint8, device=self.dev)
+            d.work = self._work.data_ptr()
+            d.work_bytes = self._work.numel()
         p = _lib.StepParams()
         p.dt, p.nsteps, p.nout = float(dt), int(nsteps), int(nout)
         p.step_offset, p.rec_capacity = int(step_offset), int(rec_capacity)

@@ -62,8 +62,21 @@ def test_argument_validation_without_gpu(lib):
     assert _call(lib, heff=None) == _lib.LIME_ERR_ARG
     assert _call(lib, n_dense=2) == _lib.LIME_ERR_ARG          # no lops
     assert _call(lib, obs=0x3000) == _lib.LIME_ERR_ARG         # obs without e_ops
-    assert _call(lib, n=97) == _lib.LIME_ERR_UNSUPPORTED
     assert _call(lib, flags=_lib.LIME_F_PREC_FP64) == _lib.LIME_ERR_UNSUPPORTED
-    assert b"n = 97" in lib.lime_last_error() or True
-    with pytest.raises(_lib.LimeUnsupported):
+    # n > 96 runs the tiled kernels, which need a caller-provided workspace
+    assert _call(lib, n=97) == _lib.LIME_ERR_ARG
+    assert b"workspace" in lib.lime_last_error()
+    assert _call(lib, n=200, work=0x4000, work_bytes=1 << 40,
+                 flags=_lib.LIME_F_RENORM_TRACE) == _lib.LIME_ERR_UNSUPPORTED
+    with pytest.raises(_lib.LimeError):
         _lib.check(_call(lib, n=200))
+
+
+def test_workspace_query(lib):
+    d = _lib.LindbladBatch()
+    d.batch, d.n, d.group, d.eop_group = 4, 64, 1, 1
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 0          # resident
+    d.flags = _lib.LIME_F_FORCE_TILED
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 64 * 64 * 8 * 3
+    d.flags, d.n, d.n_dense = 0, 130, 2
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 130 * 130 * 8 * 5

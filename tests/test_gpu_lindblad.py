@@ -165,10 +165,49 @@ def test_chunked_run_equals_single_launch():
     assert torch.equal(r1, r2) and torch.equal(o1, o2) and torch.equal(s1, s2)
 
 
-def test_unsupported_size_raises():
-    from lime_amd import _lib
-    n = 200
-    H = np.eye(n, dtype=complex)
-    prop = _prop()(H, [])
-    with pytest.raises(_lib.LimeUnsupported):
-        prop.run(np.eye(n, dtype=complex) / n, 0.01, 2)
+@pytest.mark.parametrize("n,kd,tiled", [(10, 0, True), (33, 2, True), (64, 0, True),
+                                         (65, 1, True), (130, 0, False), (150, 2, False)])
+def test_tiled_path_matches_oracle(n, kd, tiled):
+    """Per-stage tiled kernels (forced at small n, automatic above 96), incl. dense L,
+    observables (FP64 atomics) and state records."""
+    rng = np.random.default_rng(300 + n)
+    H = rand_herm(rng, n)
+    c_ops = [0.1 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(kd)]
+    c_ops += [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_dm(rng, n), rand_c(rng, n, n)])
+    A = [rand_herm(rng, n), np.diag(np.eye(n)[0]).astype(complex)]
+    dt, nt = 0.02, 12
+    prop = _prop()(H, c_ops, group=2, force_tiled=tiled)
+    rho, obs, states = prop.run(X0, dt, nt, e_ops=A, nout=3, store_states=True)
+    for b in range(2):
+        ref, ref_obs, ref_states = ol.propagate(H, X0[b], c_ops, dt, nt, A, nout=3, store_states=True)
+        scale = max(1.0, np.max(np.abs(ref)))
+        err = np.max(np.abs(rho[b].cpu().numpy() - ref)) / scale
+        print(f"tiled n={n} kd={kd} b={b} rel err {err:.3e}")
+        assert err < 2e-7
+        assert np.max(np.abs(obs[b].cpu().numpy() - ref_obs)) < 1e-6 * max(1.0, np.max(np.abs(ref_obs)))
+        assert np.max(np.abs(states[b].cpu().numpy() - np.array(ref_states))) < 2e-7 * scale
+
+
+def test_sandwich_terms_resident_and_tiled():
+    """Explicit (L, R) sandwich terms sum_k L_k X R_k^dag with R != L."""
+    from oracle import lindblad as olb
+    rng = np.random.default_rng(77)
+    for n, tiled in ((12, False), (12, True), (100, False)):
+        G = rand_c(rng, n, n) * 0.3
+        Ls = [0.2 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(2)]
+        Rs = [0.2 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(2)]
+        X0 = rand_c(rng, n, n)
+        def rhs(x, G=G, Ls=Ls, Rs=Rs):
+            out = -1j * (G @ x - x @ G.conj().T)
+            for L, R in zip(Ls, Rs):
+                out = out + L @ x @ R.conj().T
+            return out
+        ref = X0.copy()
+        for _ in range(10):
+            ref = olb.rk4(ref, rhs, 0.05)
+        prop = _prop()(G, None, sandwich=(Ls, Rs), force_tiled=tiled)
+        rho, _, _ = prop.run(X0, 0.05, 10)
+        err = np.max(np.abs(rho[0].cpu().numpy() - ref)) / np.max(np.abs(ref))
+        print(f"sandwich n={n} tiled={tiled} rel err {err:.3e}")
+        assert err < 5e-7
