@@ -32,6 +32,7 @@ from . import phys
 from . import superoperator as superop
 from .mol import Result
 from .propagator import LindbladPropagator, as_c128, device
+from .redfield import Redfield_solver, _redfield, redfield_tensor  # noqa: F401  (lime.oqs names)
 
 
 def _np(t):

@@ -1,0 +1,240 @@
+"""Bloch-Redfield propagation on the MI355X kernels (drop-in for lime/oqs.py:40-579).
+
+lime builds the Redfield tensor in the eigenbasis of H as an N^2 x N^2 Liouville matrix
+(``redfield_tensor``, oqs.py:528-579)
+
+    R = -i op2sop(diag E) - sum_k op2sop(A_k) (left(Lam_k) - right(Lam_k^dag)),
+    A_k = v^dag a_k v,  Lam_k = C_k o A_k,  C_k[n, m] = S_k(-(E_n - E_m)),
+
+and steps vec(rho) with RK4 matvecs (``_redfield``, oqs.py:373-472): O(N^4) per RHS.
+The same generator in Hilbert space is
+
+    d rho/dt = -i[E, rho] - sum_k [A_k, Lam_k rho - rho Lam_k^dag]
+             = -i (G rho - rho G^dag) + sum_k (A_k rho Lam_k^dag + Lam_k rho A_k^dag),
+    G = diag(E) - i sum_k A_k Lam_k,
+
+i.e. the Lindblad kernel's right-hand side with H_eff = G and 2K sandwich terms
+(L, R) = (A_k, Lam_k), (Lam_k, A_k): 2 + 4K complex GEMMs per RHS, O(N^3).
+``Redfield_solver.evolve`` runs exactly that on ``lime_lindblad_rk4``.
+
+``_redfield`` with an explicit Liouville tensor R keeps lime's Liouville-space algorithm
+(RK4 on vec rho, R vec rho per stage) as batched GPU products; ``propagator``/``gf``/
+``expect``/``correlation_2op_1t``/``correlation_4op_3t`` (oqs.py:145-366) evaluate the
+Liouville-space propagator U(t) on the GPU (eigendecomposition for 'SOS', RK4 for 'EOM').
+"""
+import numpy as np
+import scipy.sparse as sp
+import torch
+from scipy.linalg import eigh
+
+from . import superoperator as so
+from .mol import Result
+from .propagator import LindbladPropagator, as_c128, device
+
+
+def _dense(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    return a.toarray() if sp.issparse(a) else np.asarray(a)
+
+
+def isherm(a, tol=1e-8):
+    a = _dense(a)
+    return np.allclose(a, np.conjugate(a).T, atol=tol)
+
+
+def _spectral_matrix(spec, W):
+    """C[n, m] = S(-W[n, m]); vectorised when the callback accepts arrays."""
+    try:
+        c = np.asarray(spec(-W), dtype=np.float64)
+        if c.shape == W.shape:
+            return c
+    except Exception:
+        pass
+    return np.vectorize(lambda x: float(spec(x)))(-W)
+
+
+def bath_operators(H, a_ops, spectra):
+    """Eigen-decomposition and the eigenbasis operators (E, evecs, A_k, Lam_k)."""
+    for a in a_ops:
+        if not isherm(a):
+            raise TypeError("Operators in a_ops must be Hermitian.")
+    evals, evecs = eigh(_dense(H))
+    W = np.real(evals[:, None] - evals[None, :])
+    A = [evecs.conj().T @ _dense(a) @ evecs for a in a_ops]
+    C = [_spectral_matrix(s, W) for s in spectra]
+    Lam = [C[k] * A[k] for k in range(len(a_ops))]
+    return evals, evecs, A, Lam
+
+
+def redfield_tensor(H, a_ops, spectra, secular=False):
+    """(R as scipy CSR, evecs) exactly as lime/oqs.py:528-579 (``secular`` unused there too)."""
+    evals, evecs, A, Lam = bath_operators(H, a_ops, spectra)
+    R = -1j * so.op2sop(np.diag(evals))
+    for a, l in zip(A, Lam):
+        R = R - so.op2sop(a) @ (so.left(l) - so.right(l.conj().T))
+    return sp.csr_matrix(R), evecs
+
+
+def _redfield(R, rho0, evecs=None, Nt=1, dt=0.005, t0=0, e_ops=None, return_result=True):
+    """RK4 on vec rho with dense R on the GPU (oqs.py:373-468 semantics)."""
+    dev = device()
+    e_ops = [] if e_ops is None else list(e_ops)
+    n = _dense(rho0).shape[0]
+    if evecs is not None:
+        rho0 = evecs.conj().T @ _dense(rho0) @ evecs
+        e_ops = [evecs.conj().T @ _dense(e) @ evecs for e in e_ops]
+    Rm = as_c128(_dense(R), dev)
+    v = as_c128(_dense(rho0).flatten(), dev)
+    E = torch.stack([as_c128(e, dev).transpose(0, 1).contiguous().flatten() for e in e_ops]) if e_ops else None
+    V = as_c128(evecs, dev) if evecs is not None else None
+    obs = torch.zeros((Nt, len(e_ops)), dtype=torch.complex128, device=dev)
+    states = []
+    for k in range(Nt):
+        k1 = Rm @ v
+        k2 = Rm @ (v + 0.5 * dt * k1)
+        k3 = Rm @ (v + 0.5 * dt * k2)
+        k4 = Rm @ (v + dt * k3)
+        v = v + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        if E is not None:
+            obs[k] = E @ v
+        if return_result:
+            tmp = v.reshape(n, n)
+            states.append(V @ tmp @ V.conj().transpose(0, 1) if V is not None else tmp.clone())
+    if not return_result:
+        return _dense(v.reshape(n, n))
+    res = Result(dt=dt, Nt=Nt, rho0=rho0, t0=t0)
+    res.observables = _dense(obs)
+    res.rholist = [_dense(s) for s in states]
+    return res
+
+
+class Redfield_solver:
+    """oqs.py:40-370 with the Hilbert-space Redfield generator on the fused kernel."""
+
+    def __init__(self, H, c_ops=None, spectra=None, e_ops=None):
+        self.H = H
+        self.c_ops = c_ops
+        self.R = None
+        self.spectra = spectra
+        self.evecs = None
+        self.dim = _dense(H).shape[0]
+        self.U = None
+        self.G = None
+        self.e_ops = e_ops
+        self._ops = None
+
+    def idm(self, sp_=True):
+        v = so.dm2vec(sp.identity(self.dim)) if sp_ else so.dm2vec(np.identity(self.dim))
+        return v
+
+    def configure(self, H, c_ops, e_ops):
+        self.c_ops, self.e_ops, self.H = c_ops, e_ops, H
+        self.dim = _dense(H).shape[0]
+        self.R = self._ops = None
+
+    def redfield_tensor(self, secular=False):
+        if self.spectra is None:
+            raise TypeError("Specify the bath spectral function.")
+        R, evecs = redfield_tensor(self.H, self.c_ops, self.spectra, secular)
+        self.R, self.evecs = R, evecs
+        self._ops = bath_operators(self.H, self.c_ops, self.spectra)
+        return R, evecs
+
+    def _generator(self):
+        if self._ops is None:
+            self.redfield_tensor()
+        evals, evecs, A, Lam = self._ops
+        G = np.diag(evals).astype(np.complex128)
+        for a, l in zip(A, Lam):
+            G = G - 1j * (a @ l)
+        Ls = list(A) + list(Lam)
+        Rs = list(Lam) + list(A)
+        return G, Ls, Rs, evecs
+
+    def evolve(self, rho0, dt, Nt, evecs=None, e_ops=None, store_states=False, t0=0, nout=1):
+        """RK4 in the eigenbasis (oqs.py:66-90 -> 373-468); lime ignores ``evecs`` too."""
+        e_ops = [] if e_ops is None else list(e_ops)
+        G, Ls, Rs, V = self._generator()
+        rho_eb = V.conj().T @ _dense(rho0) @ V
+        e_eb = [V.conj().T @ _dense(e) @ V for e in e_ops]
+        prop = LindbladPropagator(G, None, sandwich=(Ls, Rs))
+        _, obs, states = prop.run(rho_eb, dt, Nt, e_ops=e_eb or None, nout=nout, store_states=True)
+        dev = prop.dev
+        Vt = as_c128(V, dev)
+        back = torch.matmul(torch.matmul(Vt, states[0]), Vt.conj().transpose(0, 1))
+        res = Result(dt=dt, Nt=Nt, rho0=rho0, t0=t0, nout=nout)
+        res.observables = _dense(obs[0]) if obs is not None else np.zeros((Nt // nout, 0), np.complex128)
+        res.rholist = list(_dense(back))
+        return res
+
+    # ---------------- Liouville-space propagator and response functions (oqs.py:145-366)
+    def propagator(self, t, method="SOS"):
+        if self.R is None:
+            raise TypeError("Redfield tensor is not computed. Please call redfield_tensor()")
+        dev = device()
+        Rm = as_c128(_dense(self.R), dev)
+        t = np.asarray(t, dtype=np.float64)
+        if method == "EOM":
+            dt = t[1] - t[0]
+            U = torch.eye(Rm.shape[0], dtype=torch.complex128, device=dev)
+            Ul = []
+            for _ in range(len(t)):
+                Ul.append(U.clone())
+                k1 = Rm @ U
+                k2 = Rm @ (U + 0.5 * dt * k1)
+                k3 = Rm @ (U + 0.5 * dt * k2)
+                k4 = Rm @ (U + dt * k3)
+                U = U + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+            Ut = torch.stack(Ul, dim=-1)
+        elif method in ("eseries", "SOS"):
+            w, U1 = torch.linalg.eig(Rm)
+            U2 = torch.linalg.inv(U1)
+            ph = torch.exp(w[:, None] * torch.as_tensor(t, device=dev).to(torch.complex128)[None, :])
+            Ut = torch.einsum("aj,jk,jb->abk", U1, ph, U2)
+        else:
+            raise ValueError(f"unknown propagator method {method!r}")
+        self.U = _dense(Ut)
+        self.G = -1j * self.U
+        return self.U
+
+    def gf(self, t, w=None, secular=False, k=1, domain="time", method="EOM"):
+        if self.R is None:
+            self.redfield_tensor(secular=secular)
+        return -1j * self.propagator(t, method="EOM" if method == "EOM" else "SOS")
+
+    def expect(self, rho0, e_ops):
+        V = self.evecs
+        rho_eb = so.dm2vec(V.conj().T @ _dense(rho0) @ V)
+        e_eb = [V.conj().T @ _dense(e) @ V for e in e_ops]
+        rho_t = np.tensordot(self.U, rho_eb, axes=([1], [0]))          # (N^2, Nt)
+        return np.stack([so.dm2vec(e) @ rho_t for e in e_eb], axis=1)
+
+    def correlation_2op_1t(self, rho0, a, b, tau):
+        if self.G is None:
+            self.propagator(tau)
+        idm = so.dm2vec(np.identity(self.dim))
+        r0 = _dense(rho0)
+        r0 = so.dm2vec(r0) if r0.ndim == 2 else r0
+        x = _dense(b) @ r0
+        return idm @ (_dense(a) @ np.tensordot(self.G, x, axes=([1], [0])))
+
+    def correlation_4op_3t(self, rho0, oplist, signature, tau):
+        """<<I| a G(t3) b G(t2) c G(t1) d |rho0>> as an (Nt, Nt, Nt) array (oqs.py:277-366)."""
+        if len(oplist) != 4:
+            raise ValueError("Number of operators is not 4.")
+        a, b, c, d = [so.operator_to_superoperator(_dense(o), s).toarray()
+                      for o, s in zip(oplist, signature)]
+        if self.G is None:
+            self.propagator(tau)
+        dev = device()
+        G = as_c128(self.G, dev)
+        idm = as_c128(so.dm2vec(np.identity(self.dim)), dev)
+        rho = as_c128(d @ so.dm2vec(_dense(rho0)), dev)
+        tmp = torch.tensordot(G, rho, dims=([1], [0]))                    # (a, k)
+        tmp = as_c128(c, dev) @ tmp
+        tmp = torch.tensordot(G, tmp, dims=([1], [0]))                    # (a, j, k)
+        tmp = torch.tensordot(as_c128(b, dev), tmp, dims=([1], [0]))
+        tmp = torch.tensordot(G, tmp, dims=([1], [0]))                    # (a, i, j, k)
+        out = torch.einsum("a,ab,bijk->ijk", idm, as_c128(a, dev), tmp)
+        return _dense(out)

@@ -66,3 +66,29 @@ def propagate(R, evecs, rho0, dt, Nt, e_ops=()):
         states[k] = evecs @ tmp @ dag(evecs)
         obs[k] = [np.trace(e @ tmp) for e in e_eb]
     return states, obs
+
+
+def propagator_expm(R, tlist):
+    """U(t) = exp(R t) for each t (scipy expm), stacked as (N^2, N^2, Nt)."""
+    from scipy.linalg import expm
+    R = np.asarray(R)
+    return np.stack([expm(R * t) for t in tlist], axis=-1)
+
+
+def response_4op_3t(R, rho0, ops_super, tlist):
+    """<<I| a G(t_i) b G(t_j) c G(t_k) d |rho0>> with G = -i exp(R t), explicit loops
+    (oqs.py:277-366 semantics, out[i, j, k])."""
+    a, b, c, d = [np.asarray(o) for o in ops_super]
+    n = int(round(np.sqrt(R.shape[0])))
+    G = -1j * propagator_expm(R, tlist)
+    idm = np.identity(n).flatten()
+    v0 = d @ np.asarray(rho0).flatten()
+    nt = len(tlist)
+    out = np.zeros((nt, nt, nt), dtype=np.complex128)
+    for k in range(nt):
+        x = c @ (G[:, :, k] @ v0)
+        for j in range(nt):
+            y = b @ (G[:, :, j] @ x)
+            for i in range(nt):
+                out[i, j, k] = idm @ (a @ (G[:, :, i] @ y))
+    return out
