@@ -35,7 +35,9 @@ extern "C" {
 #define LIME_F_PREC_MASK 0x30      /* arithmetic of the right-hand-side GEMMs:          */
 #define LIME_F_PREC_FP32 0x00      /*   complex64 operands on FP32 MFMA (default)       */
 #define LIME_F_PREC_3XBF16 0x10    /*   3-term BF16 split products on BF16 MFMA         */
-#define LIME_F_PREC_FP64 0x20      /*   complex128 operands on FP64 MFMA                */
+#define LIME_F_PREC_FP64 0x20      /*   complex128 operands on FP64 MFMA (tiled kernels;
+                                        operator sets and x/y64 of lime_lindblad_apply are
+                                        then complex128)                                  */
 #define LIME_F_FORCE_TILED 0x100   /* use the per-stage tiled kernels even when n fits the
                                       resident kernel (testing / tuning)                  */
 
@@ -95,6 +97,34 @@ typedef struct lime_step_params {
  * lime/correlation.py:17-69, which all step the same right-hand side.
  */
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream);
+
+/*
+ * One application of the generator to every operator of the batch (adjoint and stage
+ * recomputation building block):
+ *   V = f(X)   (the right-hand side above; X in the operand precision, [batch][n][n])
+ *   y   = c_lx V + c_z z   (complex128, optional; z optional)
+ *   y64 = copy of y in the operand precision (complex64, or complex128 with PREC_FP64)
+ *   acc += c_acc V         (complex128, optional)
+ * Complex scalars are passed as (re, im) pairs.  Uses d->heff/dmat/lops/rops/group and,
+ * when n_dense > 0, d->work (>= batch * n_dense * n * n * 8 bytes) for L_k X.
+ * Replaces one call of lime's liouvillian (lime/oqs.py:706-713); with the adjoint operator
+ * set (H_eff -> -H_eff^dag, D -> conj(D), (L, R) -> (L^dag, R^dag)) it applies f^dag.
+ */
+int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const void* z, void* y,
+                        void* y64, void* acc, const double* c_lx, const double* c_z,
+                        const double* c_acc, void* stream);
+
+/*
+ * Parameter-gradient contractions of the discrete RK4 adjoint: with P = conj(kappa) o s
+ * (both complex128 [batch][n][n]) accumulate, per operator,
+ *   out_a[j] += w (sum_m P_jm - sum_m P_mj)          (Im(out_a) = d/dH_jj contribution)
+ *   out_b[j] += w (P_jj - (sum_m P_jm + sum_m P_mj)/2)  (Re(out_b) = d/d gamma_j)
+ *   out_u[j] += w (coupling contraction of H_j,j+1 = H_j+1,j) (Im(out_u))
+ * out_a, out_b: complex128 [batch][n]; out_u: complex128 [batch][n-1].
+ * No counterpart in lime (it has no gradients).
+ */
+int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const void* s, double w,
+                          void* out_a, void* out_b, void* out_u, void* stream);
 
 /* Largest Hilbert dimension the resident (workspace-free) kernel accepts. */
 int lime_lindblad_max_n(int32_t n_dense, int32_t flags);

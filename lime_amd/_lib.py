@@ -26,6 +26,8 @@ EXPORTS = (
     "lime_lindblad_rk4",
     "lime_lindblad_max_n",
     "lime_lindblad_workspace_bytes",
+    "lime_lindblad_apply",
+    "lime_adjoint_contract",
     "lime_last_error",
     "lime_version",
 )
@@ -98,6 +100,13 @@ def load(path=LIB_PATH):
         lib.lime_lindblad_max_n.restype = ctypes.c_int
         lib.lime_lindblad_workspace_bytes.argtypes = [ctypes.POINTER(LindbladBatch)]
         lib.lime_lindblad_workspace_bytes.restype = ctypes.c_int64
+        vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+        lib.lime_lindblad_apply.argtypes = [ctypes.POINTER(LindbladBatch), vp, vp, vp, vp, vp,
+                                            dp, dp, dp, vp]
+        lib.lime_lindblad_apply.restype = ctypes.c_int
+        lib.lime_adjoint_contract.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp,
+                                              ctypes.c_double, vp, vp, vp, vp]
+        lib.lime_adjoint_contract.restype = ctypes.c_int
         lib.lime_last_error.argtypes = []
         lib.lime_last_error.restype = ctypes.c_char_p
         lib.lime_version.argtypes = []

@@ -1,0 +1,195 @@
+"""Exact discrete adjoint of the RK4 Lindblad/QRT propagation, behind torch.autograd.
+
+lime has no gradients; this module differentiates the linear-absorption pipeline of the
+chain model (site energies eps, nearest-neighbour couplings J, one shared pure-dephasing
+rate gamma) through the same RK4 map the forward kernels apply.
+
+Model (Hilbert space = ground state |g> = index 0 + n_sites sites):
+    H = sum_j eps_j |j><j| + J_j (|j><j+1| + h.c.),  L_j = sqrt(gamma) |j><j|  (sites only),
+    mu = sum_j mu_j (|j><g| + |g><j|),  X_0 = mu |g><g|,  C_k = Tr[mu X_k],  k = 1..Nt.
+
+Forward: the fused kernel (resident for n <= 96, tiled above) records C_k every step and
+the checkpoints X_{cK} every K steps (default K = 64).
+
+Backward, for a real loss with torch's gradient G_k = dL/dC_k^* convention
+(dL = Re sum_k conj(G_k) dC_k): the source of step k is g_k = G_k mu^dag, and with the RK4
+stages s_1 = X, s_2 = X + h/2 f(s_1), s_3 = X + h/2 f(s_2), s_4 = X + h f(s_3) of step k
+(recomputed from the checkpoint of their segment) and lambda' = lambda_{k+1}:
+    kappa_4 = h/6 lambda',               sigma_4 = f^dag(kappa_4)
+    kappa_3 = h/3 lambda' + h sigma_4,   sigma_3 = f^dag(kappa_3)
+    kappa_2 = h/3 lambda' + h/2 sigma_3, sigma_2 = f^dag(kappa_2)
+    kappa_1 = h/6 lambda' + h/2 sigma_2, sigma_1 = f^dag(kappa_1)
+    lambda_k = lambda' + sigma_1 + sigma_2 + sigma_3 + sigma_4 + g_k
+    dL/dtheta += Re sum_i <kappa_i, (df/dtheta) s_i>.
+f^dag is the generator with H_eff -> -H_eff^dag, D -> conj(D): the SAME kernels
+(``lime_lindblad_apply``); the contractions run in ``lime_adjoint_contract``.
+"""
+import numpy as np
+import torch
+
+from .propagator import LindbladPropagator, device
+
+
+def chain_generator(eps, J, gamma):
+    """H_eff, D (complex128, (B, n, n)) of the chain + ground model on the GPU."""
+    B, ns = eps.shape
+    n = ns + 1
+    dev = eps.device
+    H = torch.zeros((B, n, n), dtype=torch.complex128, device=dev)
+    idx = torch.arange(1, n, device=dev)
+    H[:, idx, idx] = eps.to(torch.complex128)
+    H[:, idx[:-1], idx[1:]] = J.to(torch.complex128)
+    H[:, idx[1:], idx[:-1]] = J.to(torch.complex128)
+    site = torch.zeros(n, dtype=torch.float64, device=dev)
+    site[1:] = 1.0
+    g = gamma.to(torch.float64).reshape(-1, 1) if torch.is_tensor(gamma) else torch.full((B, 1), float(gamma), dtype=torch.float64, device=dev)
+    if g.shape[0] == 1:
+        g = g.expand(B, 1)
+    heff = H - 0.5j * torch.diag_embed((g * site).to(torch.complex128))
+    dmat = torch.diag_embed((g * site).to(torch.complex128))
+    return heff.contiguous(), dmat.contiguous()
+
+
+def dipole_operator(mu_sites, n):
+    mu = torch.zeros((n, n), dtype=torch.complex128, device=mu_sites.device)
+    mu[1:, 0] = mu_sites.to(torch.complex128)
+    mu[0, 1:] = mu_sites.to(torch.complex128)
+    return mu
+
+
+class ChainQRT:
+    """Forward propagation with checkpoints and the adjoint backward pass."""
+
+    def __init__(self, eps, J, gamma, mu_sites, dt, nt, checkpoint=64, force_tiled=False,
+                 precision="fp32"):
+        self.dev = device()
+        self.eps = eps.to(self.dev, torch.float64)
+        self.J = J.to(self.dev, torch.float64)
+        self.gamma = gamma
+        self.B, self.ns = self.eps.shape
+        self.n = self.ns + 1
+        self.dt, self.nt, self.K = float(dt), int(nt), int(checkpoint)
+        heff, dmat = chain_generator(self.eps, self.J, torch.as_tensor(gamma, device=self.dev))
+        self.prop = LindbladPropagator.from_folded(heff, dmat, force_tiled=force_tiled,
+                                                   precision=precision)
+        self.adj = self.prop.adjoint()
+        self.mu = dipole_operator(torch.as_tensor(mu_sites, device=self.dev), self.n)
+        self.x0 = torch.zeros((self.B, self.n, self.n), dtype=torch.complex128, device=self.dev)
+        self.x0[:, :, 0] = self.mu[:, 0]
+
+    def forward(self):
+        """C (B, Nt) complex128 and the checkpoints X_0, X_K, X_2K, ..."""
+        B, n = self.B, self.n
+        X = self.x0.clone()
+        E_t = self.mu.transpose(0, 1).contiguous().reshape(1, 1, n, n)
+        obs = torch.zeros((B, self.nt, 1), dtype=torch.complex128, device=self.dev)
+        ncp = self.nt // self.K
+        cps = torch.empty((B, ncp, n, n), dtype=torch.complex128, device=self.dev) if ncp else None
+        self.x_start = X.clone()
+        done = 0
+        while done < self.nt:
+            m = min(self.K * 16, self.nt - done)
+            self.prop.step(X, self.dt, m, eops_t=E_t, eop_group=B, obs=obs, nout=1,
+                           step_offset=done, rec_capacity=self.nt)
+            done += m
+        # checkpoints: a second pass recording states every K steps (cheap next to the
+        # 2.75x backward; keeps the C(t) pass free of state traffic)
+        if ncp:
+            X2 = self.x0.clone()
+            self.prop.step(X2, self.dt, ncp * self.K, states=cps, nout=self.K, step_offset=0,
+                           rec_capacity=ncp)
+        self.checkpoints = cps
+        return obs[:, :, 0]
+
+    def _segment_states(self, c0, length):
+        """X_{c0}, ..., X_{c0+length-1} recomputed from the checkpoint at step c0."""
+        B, n = self.B, self.n
+        Xc = self.x0.clone() if c0 == 0 else self.checkpoints[:, c0 // self.K - 1].clone().contiguous()
+        out = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
+        out[:, 0] = Xc
+        if length > 1:
+            st = torch.empty((B, length - 1, n, n), dtype=torch.complex128, device=self.dev)
+            self.prop.step(Xc, self.dt, length - 1, states=st, nout=1, step_offset=0,
+                           rec_capacity=length - 1)
+            out[:, 1:] = st
+        return out
+
+    def backward(self, G):
+        """G (B, Nt) complex: torch's dL/dC^*.  Returns (g_eps (B,ns), g_J (B,ns-1), g_gamma (B,))."""
+        B, n, h = self.B, self.n, self.dt
+        dev = self.dev
+        mu_h = self.mu.conj().transpose(0, 1).contiguous()
+        G = G.to(dev, torch.complex128)
+        lam = torch.zeros((B, n, n), dtype=torch.complex128, device=dev)
+        outa = torch.zeros((B, n), dtype=torch.complex128, device=dev)
+        outb = torch.zeros((B, n), dtype=torch.complex128, device=dev)
+        outu = torch.zeros((B, max(n - 1, 1)), dtype=torch.complex128, device=dev)
+        c128 = lambda: torch.empty((B, n, n), dtype=torch.complex128, device=dev)
+        opd = self.prop.operand_dtype
+        c64 = lambda: torch.empty((B, n, n), dtype=opd, device=dev)
+        s2, s3, s4, k3, k2, k1 = c128(), c128(), c128(), c128(), c128(), c128()
+        s2l, s3l, s4l, k3l, k2l, k1l = c64(), c64(), c64(), c64(), c64(), c64()
+        lib = self.prop._lib
+        from . import _lib as L
+        stream = torch.cuda.current_stream(dev)
+        nseg = (self.nt + self.K - 1) // self.K
+        for seg in range(nseg - 1, -1, -1):
+            c0 = seg * self.K
+            length = min(self.K, self.nt - c0)
+            Xs = self._segment_states(c0, length)
+            for k in range(c0 + length - 1, c0 - 1, -1):
+                # lambda' = lambda_{k+1}: add the source of record k+1 (C_{k+1} = Tr[mu X_{k+1}])
+                lam.add_(G[:, k].reshape(B, 1, 1) * mu_h)
+                X = Xs[:, k - c0].contiguous()
+                Xl = X.to(opd).contiguous()
+                self.prop.apply(Xl, z=X, y=s2, y64=s2l, c_lx=h / 2, c_z=1.0)
+                self.prop.apply(s2l, z=X, y=s3, y64=s3l, c_lx=h / 2, c_z=1.0)
+                self.prop.apply(s3l, z=X, y=s4, y64=s4l, c_lx=h, c_z=1.0)
+                k4 = (h / 6.0) * lam
+                k4l = k4.to(opd).contiguous()
+                acc = lam.clone()
+                self.adj.apply(k4l, z=lam, y=k3, y64=k3l, acc=acc, c_lx=h, c_z=h / 3.0, c_acc=1.0)
+                self.adj.apply(k3l, z=lam, y=k2, y64=k2l, acc=acc, c_lx=h / 2, c_z=h / 3.0, c_acc=1.0)
+                self.adj.apply(k2l, z=lam, y=k1, y64=k1l, acc=acc, c_lx=h / 2, c_z=h / 6.0, c_acc=1.0)
+                self.adj.apply(k1l, acc=acc, c_acc=1.0)
+                for kap, s in ((k1, X), (k2, s2), (k3, s3), (k4, s4)):
+                    L.check(lib.lime_adjoint_contract(B, n, kap.data_ptr(), s.data_ptr(), 1.0,
+                                                      outa.data_ptr(), outb.data_ptr(),
+                                                      outu.data_ptr(), stream.cuda_stream))
+                lam = acc
+        g_eps = outa[:, 1:].imag.clone()
+        g_J = outu[:, 1:].imag.clone()
+        g_gamma = outb[:, 1:].real.sum(dim=1)
+        return g_eps, g_J, g_gamma
+
+
+class ChainAbsorption(torch.autograd.Function):
+    """C(t) of the chain + ground model as a differentiable function of (eps, J, gamma).
+
+    forward(eps (B, ns), J (B, ns-1), gamma (scalar or (B,)), mu_sites (ns,), dt, nt, K,
+            precision) -> C (B, nt) complex128.  gamma's gradient is summed over the batch
+    when gamma is a scalar (a shared rate).  precision 'fp32' runs the FP32-MFMA kernels,
+    'fp64' the FP64-MFMA tiled kernels for both passes.
+    """
+
+    @staticmethod
+    def forward(ctx, eps, J, gamma, mu_sites, dt, nt, checkpoint=64, precision="fp32"):
+        q = ChainQRT(eps.detach(), J.detach(), gamma.detach(), mu_sites, dt, nt, checkpoint,
+                     precision=precision)
+        C = q.forward()
+        ctx.q = q
+        ctx.gamma_shape = gamma.shape
+        return C
+
+    @staticmethod
+    def backward(ctx, grad_C):
+        g_eps, g_J, g_gamma = ctx.q.backward(grad_C)
+        gg = g_gamma.sum() if len(ctx.gamma_shape) == 0 else g_gamma
+        return g_eps, g_J, gg.reshape(ctx.gamma_shape), None, None, None, None, None
+
+
+def spectrum_loss(C, target, dt):
+    """L2 mismatch of |FFT| spectra (lime's e^{-i w t} convention, dt scaling)."""
+    S = torch.fft.fft(C, dim=-1) * dt
+    St = torch.fft.fft(target, dim=-1) * dt
+    return ((S.real - St.real) ** 2).sum(dim=-1)

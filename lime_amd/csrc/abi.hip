@@ -23,8 +23,11 @@ int hip_fail(hipError_t e, const char* where) {
 constexpr size_t kLdsLimit = 160 * 1024;
 
 // Does this descriptor run on the resident (single-workgroup-per-operator) kernel?
+bool is_fp64(const lime_lindblad_batch* d) { return (d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_FP64; }
+
 bool use_resident(const lime_lindblad_batch* d) {
   if (d->flags & LIME_F_FORCE_TILED) return false;
+  if (is_fp64(d)) return false;             // the FP64 variant is built for the tiled kernels
   const int nb = lime::cdiv(d->n, 16);
   if (nb > lime::resident_nb_max(d->n_dense)) return false;
   return lime::resident_lds_bytes(nb, d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
@@ -46,7 +49,7 @@ int lime_lindblad_max_n(int32_t n_dense, int32_t flags) {
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
   if (use_resident(d)) return 0;
-  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense);
+  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d));
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
@@ -64,8 +67,8 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   if ((d->obs != nullptr || d->states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: recording needs nout >= 1 and rec_capacity >= 0");
   if (p->step_offset < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: step_offset < 0");
-  if ((d->flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: only the FP32-MFMA precision is built");
+  if ((d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_3XBF16 || (d->flags & LIME_F_PREC_MASK) == 0x30)
+    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: precision flag not built");
   const hipStream_t s = static_cast<hipStream_t>(stream);
 
   if (use_resident(d)) {
@@ -98,7 +101,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   // ---- tiled path: one launch per RK4 stage, workspace supplied by the caller
   if (d->flags & LIME_F_RENORM_TRACE)
     return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: trace renormalisation is built for n <= 96 only");
-  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense);
+  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d));
   if (d->work == nullptr || d->work_bytes < (int64_t)need)
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n = " + std::to_string(d->n) + " needs a workspace of " +
                                   std::to_string(need) + " bytes (lime_lindblad_workspace_bytes)");
@@ -112,18 +115,78 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   a.flags = d->flags;
   a.rec_capacity = p->rec_capacity;
   a.dt = p->dt;
-  a.heff = static_cast<const float2*>(d->heff);
-  a.dmat = static_cast<const float2*>(d->dmat);
-  a.lops = static_cast<const float2*>(d->lops);
-  a.rops = static_cast<const float2*>(d->rops);
+  a.heff = d->heff;
+  a.dmat = d->dmat;
+  a.lops = d->lops;
+  a.rops = d->rops;
   a.eops_t = static_cast<const double2*>(d->eops_t);
   a.rho = static_cast<double2*>(d->rho);
   a.obs = static_cast<double2*>(d->obs);
   a.states = static_cast<double2*>(d->states);
   const int nout = (d->obs || d->states) ? p->nout : 0;
   hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s,
-                                 lime::tiled_pick(d->batch, d->n));
+                                 lime::tiled_pick(d->batch, d->n), is_fp64(d));
   if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (tiled) launch");
+  return LIME_OK;
+}
+
+int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const void* z, void* y,
+                        void* y64, void* acc, const double* c_lx, const double* c_z,
+                        const double* c_acc, void* stream) {
+  if (d == nullptr || x64 == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: null descriptor or x");
+  if (d->batch < 0 || d->n < 1 || d->group < 1) return fail(LIME_ERR_ARG, "lime_lindblad_apply: bad batch/n/group");
+  if (d->batch == 0) return LIME_OK;
+  if (d->heff == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: heff is required");
+  if (d->n_dense < 0 || (d->n_dense > 0 && d->lops == nullptr)) return fail(LIME_ERR_ARG, "lime_lindblad_apply: n_dense > 0 needs lops");
+  if ((y || y64) && c_lx == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: y needs c_lx");
+  if (z && c_z == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: z needs c_z");
+  if (acc && c_acc == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: acc needs c_acc");
+  if ((d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_3XBF16 || (d->flags & LIME_F_PREC_MASK) == 0x30)
+    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_apply: precision flag not built");
+  const size_t need = (size_t)d->batch * d->n_dense * d->n * d->n * (is_fp64(d) ? sizeof(double2) : sizeof(float2));
+  if (d->n_dense > 0 && (d->work == nullptr || d->work_bytes < (int64_t)need))
+    return fail(LIME_ERR_ARG, "lime_lindblad_apply: dense operators need a workspace of " + std::to_string(need) + " bytes");
+  lime::TiledArgs a = {};
+  a.batch = d->batch;
+  a.n = d->n;
+  a.group = d->group;
+  a.n_dense = d->n_dense;
+  a.heff = d->heff;
+  a.dmat = d->dmat;
+  a.lops = d->lops;
+  a.rops = d->rops;
+  a.tbuf = d->work;
+  a.s_cur = x64;
+  a.z = static_cast<const double2*>(z);
+  a.y = static_cast<double2*>(y);
+  a.y64 = y64;
+  a.acc = static_cast<double2*>(acc);
+  if (c_lx) a.c_lx = make_double2(c_lx[0], c_lx[1]);
+  if (c_z) a.c_z = make_double2(c_z[0], c_z[1]);
+  if (c_acc) a.c_acc = make_double2(c_acc[0], c_acc[1]);
+  hipError_t e = lime::tiled_apply(a, static_cast<hipStream_t>(stream), lime::tiled_pick(d->batch, d->n),
+                                   is_fp64(d));
+  if (e != hipSuccess) return hip_fail(e, "lime_lindblad_apply launch");
+  return LIME_OK;
+}
+
+int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const void* s, double w,
+                          void* out_a, void* out_b, void* out_u, void* stream) {
+  if (batch < 0 || n < 1) return fail(LIME_ERR_ARG, "lime_adjoint_contract: bad batch/n");
+  if (batch == 0) return LIME_OK;
+  if (!kappa || !s || !out_a || !out_b || (n > 1 && !out_u))
+    return fail(LIME_ERR_ARG, "lime_adjoint_contract: null pointer");
+  lime::ContractArgs c;
+  c.batch = batch;
+  c.n = n;
+  c.kappa = static_cast<const double2*>(kappa);
+  c.s = static_cast<const double2*>(s);
+  c.weight = w;
+  c.out_a = static_cast<double2*>(out_a);
+  c.out_b = static_cast<double2*>(out_b);
+  c.out_u = static_cast<double2*>(out_u);
+  hipError_t e = lime::contract_launch(c, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "lime_adjoint_contract launch");
   return LIME_OK;
 }
 

@@ -1,4 +1,5 @@
-// Lindblad RK4 for operators too large to stay resident in one CU (n > 96).
+// Lindblad RK4 for operators too large to stay resident in one CU (n > 96), and the
+// single-application kernel used by the adjoint.
 //
 // Same right-hand side as the resident kernel,
 //   f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_k L_k X R_k^dag   (R_k = L_k for Lindblad),
@@ -7,10 +8,14 @@
 //   * K-panels of 16 complex columns are staged global -> registers -> LDS (the next panel
 //     is fetched into registers while the current one feeds the MFMAs);
 //   * 4 waves in a 2 x 2 arrangement, each owning (TILE/32)^2 16x16 MFMA blocks;
-//   * the RK4 stage update runs in the epilogue: k = -i C + D o S, the FP32 increment,
-//     the next stage operand (complex64, double buffered so no tile reads a half-written
-//     operand), and at the last stage rho += increment in FP64, records (observables by
-//     FP64 atomics, states) and the next S = rho.
+//   * the RK4 stage update runs in the epilogue: k = -i C + D o S, the increment, the next
+//     stage operand (double buffered so no tile reads a half-written operand), and at the
+//     last stage rho += increment in FP64, records (observables by FP64 atomics, states)
+//     and the next S = rho.
+// Two precisions share the code (their MFMA accumulator row layouts differ, see CT::row):
+// R = float (complex64 operands, v_mfma_f32_16x16x4_f32,
+// FP32 increment) and R = double (complex128 operands, v_mfma_f64_16x16x4_f64, FP64
+// everywhere -- the FP64 GPU reference).
 // The dense products T_k = L_k S of a stage come from lindblad_left_tiled, launched first.
 #include "common.hpp"
 #include "lindblad_tiled_decl.hpp"
@@ -22,32 +27,49 @@ namespace {
 constexpr int KP = 16;          // K-panel width (complex elements)
 constexpr int KPP = KP + 1;     // LDS pitch of [row][k] panels (odd: conflict-free fragments)
 
+template <typename R> struct CT;
+template <> struct CT<float> {
+  using C = float2;
+  using V4 = floatx4;
+  __device__ static V4 mfma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static C make(double x, double y) { return make_float2((float)x, (float)y); }
+  // accumulator register v of lane group lg holds row 4*lg + v of the 16x16 block
+  __device__ static int row(int lg, int v) { return 4 * lg + v; }
+};
+template <> struct CT<double> {
+  using C = double2;
+  using V4 = doublex4;
+  __device__ static V4 mfma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static C make(double x, double y) { return make_double2(x, y); }
+  // FP64 16x16x4: register v of lane group lg holds row lg + 4*v (measured on gfx950)
+  __device__ static int row(int lg, int v) { return lg + 4 * v; }
+};
+
 // Complex accumulation modes of one product term:
 //   PLAIN : C += A B                     (B[k][j] given row-major)
 //   NEG_AH: C -= A M^dag                 (M given row-major; B[k][j] = conj(M[j][k]))
 //   I_AH  : C += i A M^dag
 enum Mode { PLAIN = 0, NEG_AH = 1, I_AH = 2 };
 
-template <int TILE>
-struct Frag {
-  static constexpr int BW = TILE / 32;   // MFMA blocks per wave per dimension
-};
-
-// Load a TILE x KP panel of a row-major n x n complex64 matrix: rows r0.., cols k0..
-template <int TILE>
-__device__ __forceinline__ void load_rows(const float2* __restrict__ M, int n, int r0, int k0,
-                                          float2 (&reg)[TILE * KP / 256], int tid) {
+// Load a TILE x KP panel of a row-major n x n matrix: rows r0.., cols k0..
+template <int TILE, typename C>
+__device__ __forceinline__ void load_rows(const C* __restrict__ M, int n, int r0, int k0,
+                                          C (&reg)[TILE * KP / 256], int tid) {
 #pragma unroll
   for (int q = 0; q < TILE * KP / 256; ++q) {
     const int e = tid + q * 256;
     const int r = e / KP, c = e % KP;
     const int gr = r0 + r, gc = k0 + c;
-    reg[q] = (gr < n && gc < n) ? M[(size_t)gr * n + gc] : make_float2(0.f, 0.f);
+    C v;
+    v.x = 0;
+    v.y = 0;
+    if (gr < n && gc < n) v = M[(size_t)gr * n + gc];
+    reg[q] = v;
   }
 }
 
-template <int TILE>
-__device__ __forceinline__ void store_rows(float2* sM, const float2 (&reg)[TILE * KP / 256], int tid) {
+template <int TILE, typename C>
+__device__ __forceinline__ void store_rows(C* sM, const C (&reg)[TILE * KP / 256], int tid) {
 #pragma unroll
   for (int q = 0; q < TILE * KP / 256; ++q) {
     const int e = tid + q * 256;
@@ -56,20 +78,24 @@ __device__ __forceinline__ void store_rows(float2* sM, const float2 (&reg)[TILE 
 }
 
 // Load a KP x TILE panel: rows k0.. (the k index), cols c0..
-template <int TILE>
-__device__ __forceinline__ void load_cols(const float2* __restrict__ M, int n, int k0, int c0,
-                                          float2 (&reg)[TILE * KP / 256], int tid) {
+template <int TILE, typename C>
+__device__ __forceinline__ void load_cols(const C* __restrict__ M, int n, int k0, int c0,
+                                          C (&reg)[TILE * KP / 256], int tid) {
 #pragma unroll
   for (int q = 0; q < TILE * KP / 256; ++q) {
     const int e = tid + q * 256;
     const int r = e / TILE, c = e % TILE;
     const int gr = k0 + r, gc = c0 + c;
-    reg[q] = (gr < n && gc < n) ? M[(size_t)gr * n + gc] : make_float2(0.f, 0.f);
+    C v;
+    v.x = 0;
+    v.y = 0;
+    if (gr < n && gc < n) v = M[(size_t)gr * n + gc];
+    reg[q] = v;
   }
 }
 
-template <int TILE>
-__device__ __forceinline__ void store_cols(float2* sM, const float2 (&reg)[TILE * KP / 256], int tid) {
+template <int TILE, typename C>
+__device__ __forceinline__ void store_cols(C* sM, const C (&reg)[TILE * KP / 256], int tid) {
 #pragma unroll
   for (int q = 0; q < TILE * KP / 256; ++q) {
     const int e = tid + q * 256;
@@ -77,18 +103,21 @@ __device__ __forceinline__ void store_cols(float2* sM, const float2 (&reg)[TILE 
   }
 }
 
+template <int TILE>
+constexpr int smem_b() { return KP * (TILE + 1) > TILE * KPP ? KP * (TILE + 1) : TILE * KPP; }
+
 // Accumulate one product term into the wave's MFMA blocks.
-//   A: row-major n x n (rows of the tile); Bsrc: row-major n x n;
-//   MODE PLAIN uses B[k][j] = Bsrc[k][j]; the other modes B[k][j] = conj(Bsrc[j][k]).
-template <int TILE, int MODE>
-__device__ __forceinline__ void gemm_term(const float2* __restrict__ A, const float2* __restrict__ Bsrc,
-                                          int n, int row0, int col0, float2* sA, float2* sB,
-                                          floatx4 (&cr)[TILE / 32][TILE / 32],
-                                          floatx4 (&ci)[TILE / 32][TILE / 32],
-                                          int tid, int wr, int wc, int lr, int lg) {
+template <int TILE, int MODE, typename R>
+__device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ A,
+                                          const typename CT<R>::C* __restrict__ Bsrc, int n, int row0,
+                                          int col0, typename CT<R>::C* sA, typename CT<R>::C* sB,
+                                          typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
+                                          typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32], int tid, int wr,
+                                          int wc, int lr, int lg) {
+  using C = typename CT<R>::C;
   constexpr int BW = TILE / 32;
   constexpr int NR = TILE * KP / 256;
-  float2 ra[NR], rb[NR];
+  C ra[NR], rb[NR];
   load_rows<TILE>(A, n, row0, 0, ra, tid);
   if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, 0, col0, rb, tid);
   else load_rows<TILE>(Bsrc, n, col0, 0, rb, tid);
@@ -105,7 +134,7 @@ __device__ __forceinline__ void gemm_term(const float2* __restrict__ A, const fl
     }
 #pragma unroll
     for (int kk = 0; kk < KP; kk += 4) {
-      float2 x[BW], y[BW];
+      C x[BW], y[BW];
 #pragma unroll
       for (int p = 0; p < BW; ++p) {
         x[p] = sA[(wr * (TILE / 2) + p * 16 + lr) * KPP + kk + lg];
@@ -116,114 +145,141 @@ __device__ __forceinline__ void gemm_term(const float2* __restrict__ A, const fl
       for (int p = 0; p < BW; ++p)
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
-          const float2 a = x[p], h = y[q];
+          const C a = x[p], h = y[q];
           if (MODE == PLAIN) {
-            cr[p][q] = mfma_f32(a.x, h.x, cr[p][q]);
-            ci[p][q] = mfma_f32(a.x, h.y, ci[p][q]);
-            cr[p][q] = mfma_f32(-a.y, h.y, cr[p][q]);
-            ci[p][q] = mfma_f32(a.y, h.x, ci[p][q]);
+            cr[p][q] = CT<R>::mfma(a.x, h.x, cr[p][q]);
+            ci[p][q] = CT<R>::mfma(a.x, h.y, ci[p][q]);
+            cr[p][q] = CT<R>::mfma(-a.y, h.y, cr[p][q]);
+            ci[p][q] = CT<R>::mfma(a.y, h.x, ci[p][q]);
           } else if (MODE == NEG_AH) {   // C -= a conj(h)
-            cr[p][q] = mfma_f32(-a.x, h.x, cr[p][q]);
-            ci[p][q] = mfma_f32(a.x, h.y, ci[p][q]);
-            cr[p][q] = mfma_f32(-a.y, h.y, cr[p][q]);
-            ci[p][q] = mfma_f32(-a.y, h.x, ci[p][q]);
+            cr[p][q] = CT<R>::mfma(-a.x, h.x, cr[p][q]);
+            ci[p][q] = CT<R>::mfma(a.x, h.y, ci[p][q]);
+            cr[p][q] = CT<R>::mfma(-a.y, h.y, cr[p][q]);
+            ci[p][q] = CT<R>::mfma(-a.y, h.x, ci[p][q]);
           } else {                       // C += i a conj(h)
-            cr[p][q] = mfma_f32(a.x, h.y, cr[p][q]);
-            ci[p][q] = mfma_f32(a.x, h.x, ci[p][q]);
-            cr[p][q] = mfma_f32(-a.y, h.x, cr[p][q]);
-            ci[p][q] = mfma_f32(a.y, h.y, ci[p][q]);
+            cr[p][q] = CT<R>::mfma(a.x, h.y, cr[p][q]);
+            ci[p][q] = CT<R>::mfma(a.x, h.x, ci[p][q]);
+            cr[p][q] = CT<R>::mfma(-a.y, h.x, cr[p][q]);
+            ci[p][q] = CT<R>::mfma(a.y, h.y, ci[p][q]);
           }
         }
     }
   }
 }
 
+__device__ __forceinline__ int tiles_per_dim(int n, int tile) { return (n + tile - 1) / tile; }
+
+// Common per-workgroup geometry.
+struct Geo {
+  int row0, col0, tid, lane, w, wr, wc, lr, lg;
+};
+
 template <int TILE>
-__device__ __forceinline__ int tiles_per_dim(int n) { return (n + TILE - 1) / TILE; }
+__device__ __forceinline__ Geo geometry(int n, int tile_index) {
+  Geo g;
+  const int tpd = tiles_per_dim(n, TILE);
+  g.row0 = (tile_index / tpd) * TILE;
+  g.col0 = (tile_index % tpd) * TILE;
+  g.tid = threadIdx.x;
+  g.lane = g.tid & 63;
+  g.w = __builtin_amdgcn_readfirstlane(g.tid >> 6);
+  g.wr = g.w >> 1;
+  g.wc = g.w & 1;
+  g.lr = g.lane & 15;
+  g.lg = g.lane >> 4;
+  return g;
+}
+
+// C = H S - S H^dag + i sum_k T_k R_k^dag for operator b (set = b / group).
+template <int TILE, typename R>
+__device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const Geo& g,
+                                                typename CT<R>::C* sA, typename CT<R>::C* sB,
+                                                typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
+                                                typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32]) {
+  using C = typename CT<R>::C;
+  const int n = a.n, set = b / a.group;
+  const size_t nn = (size_t)n * n;
+  const C* H = static_cast<const C*>(a.heff) + (size_t)set * nn;
+  const C* S = static_cast<const C*>(a.s_cur) + (size_t)b * nn;
+  gemm_term<TILE, PLAIN, R>(H, S, n, g.row0, g.col0, sA, sB, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  gemm_term<TILE, NEG_AH, R>(S, H, n, g.row0, g.col0, sA, sB, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  for (int k = 0; k < a.n_dense; ++k) {
+    const C* T = static_cast<const C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
+    const C* Rk = static_cast<const C*>(a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + k) * nn;
+    gemm_term<TILE, I_AH, R>(T, Rk, n, g.row0, g.col0, sA, sB, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  }
+}
 
 }  // namespace
 
 // T[b][k] = L[set][k] @ S[b]  for every dense operator k.
-template <int TILE>
+template <int TILE, typename R>
 __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
-  __shared__ float2 sA[TILE * KPP];
-  __shared__ float2 sB[KP * (TILE + 1) > TILE * KPP ? KP * (TILE + 1) : TILE * KPP];
-  const int tpd = tiles_per_dim<TILE>(a.n);
-  const int per = tpd * tpd;
-  const int blk = blockIdx.x;
-  const int tile = blk % per;
-  const int bk = blk / per;                   // = b * n_dense + k
+  using C = typename CT<R>::C;
+  using V4 = typename CT<R>::V4;
+  __shared__ C sA[TILE * KPP];
+  __shared__ C sB[smem_b<TILE>()];
+  const int per = tiles_per_dim(a.n, TILE) * tiles_per_dim(a.n, TILE);
+  const int bk = blockIdx.x / per;            // = b * n_dense + k
+  const Geo g = geometry<TILE>(a.n, blockIdx.x % per);
   const int b = bk / a.n_dense, k = bk % a.n_dense;
   const int set = b / a.group;
-  const int row0 = (tile / tpd) * TILE, col0 = (tile % tpd) * TILE;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = w >> 1, wc = w & 1, lr = lane & 15, lg = lane >> 4;
   const size_t nn = (size_t)a.n * a.n;
   constexpr int BW = TILE / 32;
-  floatx4 cr[BW][BW], ci[BW][BW];
+  V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
-    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-  gemm_term<TILE, PLAIN>(a.lops + ((size_t)set * a.n_dense + k) * nn, a.s_cur + (size_t)b * nn, a.n,
-                         row0, col0, sA, sB, cr, ci, tid, wr, wc, lr, lg);
-  float2* T = a.tbuf + ((size_t)b * a.n_dense + k) * nn;
+    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
+  gemm_term<TILE, PLAIN, R>(static_cast<const C*>(a.lops) + ((size_t)set * a.n_dense + k) * nn,
+                            static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, g.row0, g.col0, sA, sB,
+                            cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  C* T = static_cast<C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = row0 + wr * (TILE / 2) + p * 16 + 4 * lg + v;
-        const int j = col0 + wc * (TILE / 2) + q * 16 + lr;
-        if (i < a.n && j < a.n) T[(size_t)i * a.n + j] = make_float2(cr[p][q][v], ci[p][q][v]);
+        const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+        const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+        if (i < a.n && j < a.n) T[(size_t)i * a.n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
       }
 }
 
 // One RK4 stage for every operator of the batch.
-template <int TILE>
+template <int TILE, typename R>
 __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
-  __shared__ float2 sA[TILE * KPP];
-  __shared__ float2 sB[KP * (TILE + 1) > TILE * KPP ? KP * (TILE + 1) : TILE * KPP];
+  using C = typename CT<R>::C;
+  using V4 = typename CT<R>::V4;
+  __shared__ C sA[TILE * KPP];
+  __shared__ C sB[smem_b<TILE>()];
   __shared__ double2 sRed[4];
   const int n = a.n;
-  const int tpd = tiles_per_dim<TILE>(n);
-  const int per = tpd * tpd;
-  const int b = blockIdx.x / per, tile = blockIdx.x % per;
+  const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int b = blockIdx.x / per;
+  const Geo g = geometry<TILE>(n, blockIdx.x % per);
   const int set = b / a.group;
-  const int row0 = (tile / tpd) * TILE, col0 = (tile % tpd) * TILE;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = w >> 1, wc = w & 1, lr = lane & 15, lg = lane >> 4;
   const size_t nn = (size_t)n * n;
   constexpr int BW = TILE / 32;
 
-  floatx4 cr[BW][BW], ci[BW][BW];
+  V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
-    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-  const float2* H = a.heff + (size_t)set * nn;
-  const float2* S = a.s_cur + (size_t)b * nn;
-  gemm_term<TILE, PLAIN>(H, S, n, row0, col0, sA, sB, cr, ci, tid, wr, wc, lr, lg);
-  gemm_term<TILE, NEG_AH>(S, H, n, row0, col0, sA, sB, cr, ci, tid, wr, wc, lr, lg);
-  for (int k = 0; k < a.n_dense; ++k) {
-    const float2* T = a.tbuf + ((size_t)b * a.n_dense + k) * nn;
-    const float2* R = (a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + k) * nn;
-    gemm_term<TILE, I_AH>(T, R, n, row0, col0, sA, sB, cr, ci, tid, wr, wc, lr, lg);
-  }
+    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
+  generator_gemms<TILE, R>(a, b, g, sA, sB, cr, ci);
 
   // ---- RK4 stage epilogue
   const int st = a.stage;
   const double dt = a.dt;
-  const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
+  const R cacc = (R)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
   const double cnext = (st == 2) ? dt : 0.5 * dt;
+  const C* S = static_cast<const C*>(a.s_cur) + (size_t)b * nn;
   double2* rho = a.rho + (size_t)b * nn;
-  float2* inc = a.inc + (size_t)b * nn;
-  float2* Sn = a.s_next + (size_t)b * nn;
-  const float2* Dm = a.dmat ? a.dmat + (size_t)set * nn : nullptr;
+  C* inc = static_cast<C*>(a.inc) + (size_t)b * nn;
+  C* Sn = static_cast<C*>(a.s_next) + (size_t)b * nn;
+  const C* Dm = a.dmat ? static_cast<const C*>(a.dmat) + (size_t)set * nn : nullptr;
   const bool rec = (st == 3) && a.rec >= 0;
   double2 rfin[BW][BW][4];
 #pragma unroll
@@ -233,29 +289,35 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         rfin[p][q][v] = make_double2(0.0, 0.0);
-        const int i = row0 + wr * (TILE / 2) + p * 16 + 4 * lg + v;
-        const int j = col0 + wc * (TILE / 2) + q * 16 + lr;
+        const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+        const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
         if (i >= n || j >= n) continue;
         const size_t o = (size_t)i * n + j;
-        float kr = ci[p][q][v], ki = -cr[p][q][v];
+        R kr = ci[p][q][v], ki = -cr[p][q][v];
         if (Dm) {
-          const float2 d = Dm[o], x = S[o];
+          const C d = Dm[o], x = S[o];
           kr += d.x * x.x - d.y * x.y;
           ki += d.x * x.y + d.y * x.x;
         }
-        float2 ic = (st == 0) ? make_float2(0.f, 0.f) : inc[o];
-        ic.x = fmaf(cacc, kr, ic.x);
-        ic.y = fmaf(cacc, ki, ic.y);
+        C ic;
+        if (st == 0) {
+          ic.x = 0;
+          ic.y = 0;
+        } else {
+          ic = inc[o];
+        }
+        ic.x += cacc * kr;
+        ic.y += cacc * ki;
         double2 r = rho[o];
         if (st < 3) {
           inc[o] = ic;
-          Sn[o] = make_float2((float)(r.x + cnext * (double)kr), (float)(r.y + cnext * (double)ki));
+          Sn[o] = CT<R>::make(r.x + cnext * (double)kr, r.y + cnext * (double)ki);
         } else {
           r.x += (double)ic.x;
           r.y += (double)ic.y;
           rho[o] = r;
           rfin[p][q][v] = r;
-          Sn[o] = make_float2((float)r.x, (float)r.y);
+          Sn[o] = CT<R>::make(r.x, r.y);
           if (rec && a.states) a.states[((size_t)b * a.rec_capacity + a.rec) * nn + o] = r;
         }
       }
@@ -269,8 +331,8 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
         for (int q = 0; q < BW; ++q)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int i = row0 + wr * (TILE / 2) + p * 16 + 4 * lg + v;
-            const int j = col0 + wc * (TILE / 2) + q * 16 + lr;
+            const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+            const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
             if (i >= n || j >= n) continue;
             const double2 ev = E0[(size_t)e * nn + (size_t)i * n + j];
             pr += ev.x * rfin[p][q][v].x - ev.y * rfin[p][q][v].y;
@@ -278,9 +340,9 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
           }
       const double sr = wave_sum(pr), si = wave_sum(pi);
       __syncthreads();
-      if (lane == 0) sRed[w] = make_double2(sr, si);
+      if (g.lane == 0) sRed[g.w] = make_double2(sr, si);
       __syncthreads();
-      if (tid == 0) {
+      if (g.tid == 0) {
         double2 t = make_double2(0.0, 0.0);
         for (int x = 0; x < 4; ++x) {
           t.x += sRed[x].x;
@@ -294,8 +356,137 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
   }
 }
 
-template <int TILE>
-static hipError_t run_step(TiledArgs a, hipStream_t stream, float2* s0, float2* s1) {
+// One application of the generator to every operator of the batch, with a flexible
+// epilogue: V = f(s_cur); y = c_lx V + c_z z; y_op = y in the operand precision;
+// acc += c_acc V.
+template <int TILE, typename R>
+__global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
+  using C = typename CT<R>::C;
+  using V4 = typename CT<R>::V4;
+  __shared__ C sA[TILE * KPP];
+  __shared__ C sB[smem_b<TILE>()];
+  const int n = a.n;
+  const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int b = blockIdx.x / per;
+  const Geo g = geometry<TILE>(n, blockIdx.x % per);
+  const int set = b / a.group;
+  const size_t nn = (size_t)n * n;
+  constexpr int BW = TILE / 32;
+  V4 cr[BW][BW], ci[BW][BW];
+#pragma unroll
+  for (int p = 0; p < BW; ++p)
+#pragma unroll
+    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
+  generator_gemms<TILE, R>(a, b, g, sA, sB, cr, ci);
+  const C* Dm = a.dmat ? static_cast<const C*>(a.dmat) + (size_t)set * nn : nullptr;
+  const C* X = static_cast<const C*>(a.s_cur);
+  C* yop = static_cast<C*>(a.y64);
+#pragma unroll
+  for (int p = 0; p < BW; ++p)
+#pragma unroll
+    for (int q = 0; q < BW; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+        const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+        if (i >= n || j >= n) continue;
+        const size_t o = (size_t)b * nn + (size_t)i * n + j;
+        double vr = (double)ci[p][q][v], vi = -(double)cr[p][q][v];
+        if (Dm) {
+          const C d = Dm[(size_t)i * n + j], x = X[o];
+          vr += (double)(d.x * x.x - d.y * x.y);
+          vi += (double)(d.x * x.y + d.y * x.x);
+        }
+        if (a.y || yop) {
+          double yr = a.c_lx.x * vr - a.c_lx.y * vi;
+          double yi = a.c_lx.x * vi + a.c_lx.y * vr;
+          if (a.z) {
+            const double2 zz = a.z[o];
+            yr += a.c_z.x * zz.x - a.c_z.y * zz.y;
+            yi += a.c_z.x * zz.y + a.c_z.y * zz.x;
+          }
+          if (a.y) a.y[o] = make_double2(yr, yi);
+          if (yop) yop[o] = CT<R>::make(yr, yi);
+        }
+        if (a.acc) {
+          double2 t = a.acc[o];
+          t.x += a.c_acc.x * vr - a.c_acc.y * vi;
+          t.y += a.c_acc.x * vi + a.c_acc.y * vr;
+          a.acc[o] = t;
+        }
+      }
+}
+
+// Gradient contractions of an adjoint stage operator kappa with a forward stage operator
+// s (P = conj(kappa) o s, per operator b):
+//   out_a[j] += w (sum_m P_jm - sum_m P_mj)                -> d/dH_jj   (times -i)
+//   out_b[j] += w (P_jj - (sum_m P_jm + sum_m P_mj) / 2)    -> d/d(dephasing rate of j)
+//   out_u[j] += w (sum_m conj(k_jm) s_j+1,m + conj(k_j+1,m) s_jm
+//                  - sum_m conj(k_m,j+1) s_mj + conj(k_mj) s_m,j+1)   -> d/dH_j,j+1 (times -i)
+// One workgroup per operator; thread t owns index j = t, t + 256, ...
+__global__ void __launch_bounds__(256) adjoint_contract(ContractArgs c) {
+  const int b = blockIdx.x, n = c.n;
+  const size_t nn = (size_t)n * n;
+  const double2* K = c.kappa + (size_t)b * nn;
+  const double2* S = c.s + (size_t)b * nn;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double rr = 0, ri = 0, cr_ = 0, ci_ = 0, ur = 0, ui = 0;
+    for (int m = 0; m < n; ++m) {
+      const double2 k1 = K[(size_t)j * n + m], s1 = S[(size_t)j * n + m];   // row j
+      rr += k1.x * s1.x + k1.y * s1.y;
+      ri += k1.x * s1.y - k1.y * s1.x;
+      const double2 k2 = K[(size_t)m * n + j], s2 = S[(size_t)m * n + j];   // column j
+      cr_ += k2.x * s2.x + k2.y * s2.y;
+      ci_ += k2.x * s2.y - k2.y * s2.x;
+      if (j + 1 < n) {
+        const double2 s3 = S[(size_t)(j + 1) * n + m];                     // conj(k_jm) s_j+1,m
+        const double2 k4 = K[(size_t)(j + 1) * n + m];                     // conj(k_j+1,m) s_jm
+        const double2 k5 = K[(size_t)m * n + j + 1];                       // conj(k_m,j+1) s_mj
+        const double2 s6 = S[(size_t)m * n + j + 1];                       // conj(k_mj) s_m,j+1
+        ur += (k1.x * s3.x + k1.y * s3.y) + (k4.x * s1.x + k4.y * s1.y)
+            - (k5.x * s2.x + k5.y * s2.y) - (k2.x * s6.x + k2.y * s6.y);
+        ui += (k1.x * s3.y - k1.y * s3.x) + (k4.x * s1.y - k4.y * s1.x)
+            - (k5.x * s2.y - k5.y * s2.x) - (k2.x * s6.y - k2.y * s6.x);
+      }
+    }
+    const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
+    const double dr = kd.x * sd.x + kd.y * sd.y, di = kd.x * sd.y - kd.y * sd.x;
+    double2* A = c.out_a + (size_t)b * n + j;
+    A->x += c.weight * (rr - cr_);
+    A->y += c.weight * (ri - ci_);
+    double2* B = c.out_b + (size_t)b * n + j;
+    B->x += c.weight * (dr - 0.5 * (rr + cr_));
+    B->y += c.weight * (di - 0.5 * (ri + ci_));
+    if (j + 1 < n) {
+      double2* U = c.out_u + (size_t)b * (n - 1) + j;
+      U->x += c.weight * ur;
+      U->y += c.weight * ui;
+    }
+  }
+}
+
+hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
+  hipLaunchKernelGGL(adjoint_contract, dim3(c.batch), dim3(256), 0, stream, c);
+  return hipGetLastError();
+}
+
+template <int TILE, typename R>
+static hipError_t apply_impl(TiledArgs a, hipStream_t stream) {
+  const int tpd = (a.n + TILE - 1) / TILE;
+  const int per = tpd * tpd;
+  if (a.n_dense > 0)
+    hipLaunchKernelGGL((lindblad_left_tiled<TILE, R>), dim3(per * a.batch * a.n_dense), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((lindblad_apply_tiled<TILE, R>), dim3(per * a.batch), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64) {
+  if (fp64) return tile == 64 ? apply_impl<64, double>(a, stream) : apply_impl<32, double>(a, stream);
+  return tile == 64 ? apply_impl<64, float>(a, stream) : apply_impl<32, float>(a, stream);
+}
+
+template <int TILE, typename R>
+static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) {
   const int tpd = (a.n + TILE - 1) / TILE;
   const int per = tpd * tpd;
   for (int st = 0; st < 4; ++st) {
@@ -303,22 +494,29 @@ static hipError_t run_step(TiledArgs a, hipStream_t stream, float2* s0, float2* 
     a.s_cur = (st % 2 == 0) ? s0 : s1;
     a.s_next = (st % 2 == 0) ? s1 : s0;
     if (a.n_dense > 0) {
-      hipLaunchKernelGGL(lindblad_left_tiled<TILE>, dim3(per * a.batch * a.n_dense), dim3(256), 0,
+      hipLaunchKernelGGL((lindblad_left_tiled<TILE, R>), dim3(per * a.batch * a.n_dense), dim3(256), 0,
                          stream, a);
     }
-    hipLaunchKernelGGL(lindblad_stage_tiled<TILE>, dim3(per * a.batch), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((lindblad_stage_tiled<TILE, R>), dim3(per * a.batch), dim3(256), 0, stream, a);
   }
   return hipGetLastError();
 }
 
-__global__ void init_stage_operand(const double2* __restrict__ rho, float2* __restrict__ s, size_t count) {
+template <typename C>
+__global__ void init_stage_operand(const double2* __restrict__ rho, C* __restrict__ s, size_t count) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) s[i] = make_float2((float)rho[i].x, (float)rho[i].y);
+  if (i < count) {
+    C v;
+    v.x = rho[i].x;
+    v.y = rho[i].y;
+    s[i] = v;
+  }
 }
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense) {
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64) {
   const size_t nn = (size_t)n * n;
-  return (size_t)batch * nn * sizeof(float2) * (3 + (size_t)n_dense);   // S0, S1, inc, T_k
+  const size_t elem = fp64 ? sizeof(double2) : sizeof(float2);
+  return (size_t)batch * nn * elem * (3 + (size_t)n_dense);   // S0, S1, inc, T_k
 }
 
 int tiled_pick(int batch, int n) {
@@ -326,25 +524,35 @@ int tiled_pick(int batch, int n) {
   return ((size_t)batch * t64 * t64 >= 512) ? 64 : 32;
 }
 
-hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
-                     int tile) {
+template <typename R>
+static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
+                          int tile) {
+  using C = typename CT<R>::C;
   const size_t nn = (size_t)a.n * a.n;
-  float2* s0 = static_cast<float2*>(work);
-  float2* s1 = s0 + (size_t)a.batch * nn;
-  a.inc = s1 + (size_t)a.batch * nn;
-  a.tbuf = a.inc + (size_t)a.batch * nn;
+  C* s0 = static_cast<C*>(work);
+  C* s1 = s0 + (size_t)a.batch * nn;
+  C* inc = s1 + (size_t)a.batch * nn;
+  a.inc = inc;
+  a.tbuf = inc + (size_t)a.batch * nn;
   const size_t cnt = (size_t)a.batch * nn;
-  hipLaunchKernelGGL(init_stage_operand, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream,
-                     a.rho, s0, cnt);
+  hipLaunchKernelGGL(init_stage_operand<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, a.rho,
+                     s0, cnt);
   for (int s = 0; s < nsteps; ++s) {
-    const int g = step_offset + s + 1;
+    const int gstep = step_offset + s + 1;
     a.rec = -1;
-    if (nout > 0 && (a.obs || a.states) && g % nout == 0 && g / nout - 1 < a.rec_capacity) a.rec = g / nout - 1;
+    if (nout > 0 && (a.obs || a.states) && gstep % nout == 0 && gstep / nout - 1 < a.rec_capacity)
+      a.rec = gstep / nout - 1;
     // stage 3 writes S = rho into s0 (stages alternate s0 -> s1 -> s0 -> s1 -> s0)
-    hipError_t e = (tile == 64) ? run_step<64>(a, stream, s0, s1) : run_step<32>(a, stream, s0, s1);
+    hipError_t e = (tile == 64) ? run_step<64, R>(a, stream, s0, s1) : run_step<32, R>(a, stream, s0, s1);
     if (e != hipSuccess) return e;
   }
   return hipGetLastError();
+}
+
+hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
+                     int tile, bool fp64) {
+  return fp64 ? run_all<double>(a, nsteps, step_offset, nout, work, stream, tile)
+              : run_all<float>(a, nsteps, step_offset, nout, work, stream, tile);
 }
 
 }  // namespace lime

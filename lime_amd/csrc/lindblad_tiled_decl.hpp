@@ -10,24 +10,45 @@ struct TiledArgs {
   int batch, n, group, n_dense, n_eops, eop_group, flags;
   int stage, rec, rec_capacity;
   double dt;
-  const float2* heff;
-  const float2* dmat;
-  const float2* lops;
-  const float2* rops;      // right factors of the sandwich terms (NULL: R = L)
+  // operator sets and workspace are complex64 (FP32 variant) or complex128 (FP64 variant)
+  const void* heff;
+  const void* dmat;
+  const void* lops;
+  const void* rops;        // right factors of the sandwich terms (NULL: R = L)
   const double2* eops_t;
   double2* rho;
   double2* obs;
   double2* states;
   // workspace
-  const float2* s_cur;
-  float2* s_next;
-  float2* inc;
-  float2* tbuf;
+  const void* s_cur;
+  void* s_next;
+  void* inc;
+  void* tbuf;
+  // generator application (lindblad_apply_tiled): V = L(s_cur);
+  //   y = c_lx V + c_z z (complex128), y64 = copy of y in the operand precision,
+  //   acc += c_acc V
+  const double2* z;
+  double2* y;
+  void* y64;
+  double2* acc;
+  double2 c_lx, c_z, c_acc;
 };
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense);
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64);
 int tiled_pick(int batch, int n);
 hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
-                     int tile);
+                     int tile, bool fp64);
+hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64);
+
+struct ContractArgs {
+  int batch, n;
+  const double2* kappa;   // adjoint stage operator  [batch][n][n]
+  const double2* s;       // forward stage operator  [batch][n][n]
+  double weight;
+  double2* out_a;         // [batch][n]   += w (rowsum - colsum) of conj(kappa) o s
+  double2* out_b;         // [batch][n]   += w (diag - (rowsum + colsum)/2)
+  double2* out_u;         // [batch][n-1] += w (nearest-neighbour coupling contraction)
+};
+hipError_t contract_launch(const ContractArgs& c, hipStream_t stream);
 
 }  // namespace lime

@@ -134,9 +134,85 @@ class LindbladPropagator:
         self.n_dense = 0 if self.lops is None else self.lops.shape[1]
         self._lib = _lib.load()
 
+    @classmethod
+    def from_folded(cls, heff, dmat=None, lops=None, rops=None, group=1, precision="fp32",
+                    force_tiled=False):
+        """Build directly from H_eff (S,N,N), D (S,N,N), dense L / R (S,K,N,N) tensors."""
+        self = object.__new__(cls)
+        self.dev = device()
+        self.precision = precision
+        self.force_tiled = bool(force_tiled)
+        self._work = None
+        self._lib = _lib.load()
+        cdt = torch.complex128 if precision == "fp64" else torch.complex64
+        heff = as_c128(heff, self.dev)
+        self.nsets, self.n = heff.shape[0], heff.shape[-1]
+        self.group = int(group)
+        self.heff_c128 = heff
+        self.heff = heff.to(cdt).contiguous()
+        self.dmat = as_c128(dmat, self.dev).to(cdt).contiguous() if dmat is not None else None
+        self.lops = as_c128(lops, self.dev).to(cdt).contiguous() if lops is not None else None
+        self.rops = as_c128(rops, self.dev).to(cdt).contiguous() if rops is not None else None
+        self.n_dense = 0 if self.lops is None else self.lops.shape[1]
+        return self
+
     @property
     def batch(self):
         return self.nsets * self.group
+
+    def adjoint(self):
+        """The propagator of the adjoint generator f^dag (same batch layout):
+        H_eff -> -H_eff^dag, D -> conj(D), (L, R) -> (L^dag, R^dag)."""
+        adj = object.__new__(LindbladPropagator)
+        adj.__dict__.update(self.__dict__)
+        adj._work = None
+        h = lambda t: t.conj().transpose(-1, -2).contiguous()
+        adj.heff = (-h(self.heff)).contiguous()
+        adj.heff_c128 = (-h(self.heff_c128)).contiguous()
+        adj.dmat = self.dmat.conj().contiguous() if self.dmat is not None else None
+        adj.lops = h(self.lops) if self.lops is not None else None
+        # R = L (rops None) stays implicit: the adjoint's right factors are adj.lops too
+        adj.rops = h(self.rops) if self.rops is not None else None
+        return adj
+
+    def _descriptor(self, B):
+        d = _lib.LindbladBatch()
+        d.batch, d.n, d.group, d.n_dense = B, self.n, self.group, self.n_dense
+        d.eop_group = B
+        d.flags = _PREC_FLAGS[self.precision]
+        d.heff = self.heff.data_ptr()
+        d.dmat = self.dmat.data_ptr() if self.dmat is not None else None
+        d.lops = self.lops.data_ptr() if self.lops is not None else None
+        d.rops = self.rops.data_ptr() if self.rops is not None else None
+        return d
+
+    @property
+    def operand_dtype(self):
+        return torch.complex128 if self.precision == "fp64" else torch.complex64
+
+    def apply(self, x64, *, z=None, y=None, y64=None, acc=None, c_lx=1.0, c_z=1.0, c_acc=1.0,
+              stream=None):
+        """One generator application V = f(x) (x in the operand precision) with the epilogue
+        y = c_lx V + c_z z, y64 = y in the operand precision, acc += c_acc V (all optional)."""
+        B = x64.shape[0]
+        op_dtype = torch.complex128 if self.precision == "fp64" else torch.complex64
+        if x64.dtype != op_dtype or not x64.is_contiguous():
+            raise ValueError(f"x must be a contiguous {op_dtype} tensor (operand precision)")
+        for t in (z, y, acc):
+            if t is not None and (t.dtype != torch.complex128 or not t.is_contiguous()):
+                raise ValueError("z, y, acc must be contiguous complex128 tensors")
+        d = self._descriptor(B)
+        if self.n_dense > 0:
+            need = B * self.n_dense * self.n * self.n * (16 if self.precision == "fp64" else 8)
+            if self._work is None or self._work.numel() < need:
+                self._work = torch.empty(need, dtype=torch.uint8, device=self.dev)
+            d.work, d.work_bytes = self._work.data_ptr(), self._work.numel()
+        pair = lambda c: (ctypes.c_double * 2)(complex(c).real, complex(c).imag)
+        s = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        _lib.check(self._lib.lime_lindblad_apply(
+            ctypes.byref(d), x64.data_ptr(), ptr(z), ptr(y), ptr(y64), ptr(acc),
+            pair(c_lx), pair(c_z), pair(c_acc), ctypes.c_void_p(s.cuda_stream)))
 
     def max_n(self):
         return self._lib.lime_lindblad_max_n(self.n_dense, _PREC_FLAGS[self.precision])

@@ -38,6 +38,9 @@ def test_version_and_limits(lib):
     assert b"gfx950" in lib.lime_version()
     assert lib.lime_lindblad_max_n(0, _lib.LIME_F_PREC_FP32) == 96
     assert lib.lime_lindblad_max_n(3, _lib.LIME_F_PREC_FP32) == 80
+    d = _lib.LindbladBatch()
+    d.batch, d.n, d.group, d.eop_group, d.flags = 2, 20, 1, 1, _lib.LIME_F_PREC_FP64
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 2 * 400 * 16 * 3
 
 
 def _call(lib, **kw):
@@ -62,7 +65,8 @@ def test_argument_validation_without_gpu(lib):
     assert _call(lib, heff=None) == _lib.LIME_ERR_ARG
     assert _call(lib, n_dense=2) == _lib.LIME_ERR_ARG          # no lops
     assert _call(lib, obs=0x3000) == _lib.LIME_ERR_ARG         # obs without e_ops
-    assert _call(lib, flags=_lib.LIME_F_PREC_FP64) == _lib.LIME_ERR_UNSUPPORTED
+    assert _call(lib, flags=_lib.LIME_F_PREC_FP64) == _lib.LIME_ERR_ARG   # tiled: needs workspace
+    assert _call(lib, flags=0x30) == _lib.LIME_ERR_UNSUPPORTED
     # n > 96 runs the tiled kernels, which need a caller-provided workspace
     assert _call(lib, n=97) == _lib.LIME_ERR_ARG
     assert b"workspace" in lib.lime_last_error()

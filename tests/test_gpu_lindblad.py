@@ -211,3 +211,22 @@ def test_sandwich_terms_resident_and_tiled():
         err = np.max(np.abs(rho[0].cpu().numpy() - ref)) / np.max(np.abs(ref))
         print(f"sandwich n={n} tiled={tiled} rel err {err:.3e}")
         assert err < 5e-7
+
+
+@pytest.mark.parametrize("n,kd", [(20, 0), (70, 2), (130, 1)])
+def test_fp64_tiled_variant_matches_oracle(n, kd):
+    """The FP64-MFMA tiled kernels (FP64 GPU reference path)."""
+    rng = np.random.default_rng(500 + n)
+    H = rand_herm(rng, n)
+    c_ops = [0.1 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(kd)]
+    c_ops += [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = rand_dm(rng, n)
+    A = [rand_herm(rng, n)]
+    dt, nt = 0.02, 10
+    prop = _prop()(H, c_ops, precision="fp64")
+    rho, obs, _ = prop.run(rho0, dt, nt, e_ops=A, nout=2)
+    ref, ref_obs, _ = ol.propagate(H, rho0, c_ops, dt, nt, A, nout=2)
+    err = np.max(np.abs(rho[0].cpu().numpy() - ref))
+    print(f"fp64 n={n} kd={kd} err {err:.3e}")
+    assert err < 1e-13
+    assert np.max(np.abs(obs[0].cpu().numpy() - ref_obs)) < 1e-12
