@@ -64,7 +64,7 @@ typedef struct lime_lindblad_batch {
   int32_t n_eops;      /* observables per operator                                     */
   int32_t eop_group;   /* operators per observable set (>= 1)                          */
   int32_t flags;       /* LIME_F_*                                                     */
-  int32_t reserved;
+  int32_t variant;     /* launch-shape variant (0 = default; see lime_lindblad_variants) */
   const void* heff;    /* complex64  [sets][n][n]  (complex128 if PREC_FP64)           */
   const void* dmat;    /* complex64  [sets][n][n]  or NULL (complex128 if PREC_FP64)   */
   const void* lops;    /* complex64  [sets][n_dense][n][n] or NULL (c128 if PREC_FP64) */
@@ -128,6 +128,10 @@ int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const voi
 
 /* Largest Hilbert dimension the resident (workspace-free) kernel accepts. */
 int lime_lindblad_max_n(int32_t n_dense, int32_t flags);
+
+/* Number of launch-shape variants for this descriptor (the autotuner times each one;
+ * any value in [0, count) is valid in lime_lindblad_batch.variant). */
+int lime_lindblad_variants(const lime_lindblad_batch* d);
 
 /* Workspace bytes lime_lindblad_rk4 needs for this descriptor (0 when none). */
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d);

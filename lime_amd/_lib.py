@@ -26,6 +26,7 @@ EXPORTS = (
     "lime_lindblad_rk4",
     "lime_lindblad_max_n",
     "lime_lindblad_workspace_bytes",
+    "lime_lindblad_variants",
     "lime_lindblad_apply",
     "lime_adjoint_contract",
     "lime_last_error",
@@ -54,7 +55,7 @@ class LindbladBatch(ctypes.Structure):
         ("n_eops", ctypes.c_int32),
         ("eop_group", ctypes.c_int32),
         ("flags", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
         ("heff", ctypes.c_void_p),
         ("dmat", ctypes.c_void_p),
         ("lops", ctypes.c_void_p),
@@ -100,6 +101,8 @@ def load(path=LIB_PATH):
         lib.lime_lindblad_max_n.restype = ctypes.c_int
         lib.lime_lindblad_workspace_bytes.argtypes = [ctypes.POINTER(LindbladBatch)]
         lib.lime_lindblad_workspace_bytes.restype = ctypes.c_int64
+        lib.lime_lindblad_variants.argtypes = [ctypes.POINTER(LindbladBatch)]
+        lib.lime_lindblad_variants.restype = ctypes.c_int
         vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
         lib.lime_lindblad_apply.argtypes = [ctypes.POINTER(LindbladBatch), vp, vp, vp, vp, vp,
                                             dp, dp, dp, vp]

@@ -25,12 +25,25 @@ constexpr size_t kLdsLimit = 160 * 1024;
 // Does this descriptor run on the resident (single-workgroup-per-operator) kernel?
 bool is_fp64(const lime_lindblad_batch* d) { return (d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_FP64; }
 
+// Resident launch-shape variant requested by d->variant (0 = default).
+int resident_variant(const lime_lindblad_batch* d) {
+  const int nb = lime::cdiv(d->n, 16);
+  return (d->variant > 0 && d->variant < lime::resident_variants(nb)) ? d->variant : 0;
+}
+
 bool use_resident(const lime_lindblad_batch* d) {
   if (d->flags & LIME_F_FORCE_TILED) return false;
   if (is_fp64(d)) return false;             // the FP64 variant is built for the tiled kernels
   const int nb = lime::cdiv(d->n, 16);
   if (nb > lime::resident_nb_max(d->n_dense)) return false;
-  return lime::resident_lds_bytes(nb, d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
+  return lime::resident_lds_bytes(nb, resident_variant(d), d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
+}
+
+// Tiled tile edge requested by d->variant (1 -> 32, 2 -> 64), else the heuristic.
+int tiled_tile(const lime_lindblad_batch* d) {
+  if (d->variant == 1) return 32;
+  if (d->variant == 2) return 64;
+  return lime::tiled_pick(d->batch, d->n);
 }
 
 }  // namespace
@@ -44,6 +57,14 @@ const char* lime_version(void) { return "lime_hip 0.2 (gfx950)"; }
 int lime_lindblad_max_n(int32_t n_dense, int32_t flags) {
   if ((flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32) return 0;
   return 16 * lime::resident_nb_max(n_dense);
+}
+
+int lime_lindblad_variants(const lime_lindblad_batch* d) {
+  if (d == nullptr || d->n < 1) return 0;
+  lime_lindblad_batch probe = *d;
+  probe.variant = 0;
+  if (use_resident(&probe)) return lime::resident_variants(lime::cdiv(d->n, 16));
+  return 3;   // tiled: 0 = heuristic, 1 = 32x32 tiles, 2 = 64x64 tiles
 }
 
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
@@ -93,7 +114,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
     a.nout = (d->obs || d->states) ? p->nout : 0;
     a.step_offset = p->step_offset;
     a.rec_capacity = p->rec_capacity;
-    hipError_t e = lime::resident_launch(a, lime::cdiv(d->n, 16), s);
+    hipError_t e = lime::resident_launch(a, lime::cdiv(d->n, 16), resident_variant(d), s);
     if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (resident) launch");
     return LIME_OK;
   }
@@ -124,8 +145,8 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   a.obs = static_cast<double2*>(d->obs);
   a.states = static_cast<double2*>(d->states);
   const int nout = (d->obs || d->states) ? p->nout : 0;
-  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s,
-                                 lime::tiled_pick(d->batch, d->n), is_fp64(d));
+  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s, tiled_tile(d),
+                                 is_fp64(d));
   if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (tiled) launch");
   return LIME_OK;
 }
@@ -164,8 +185,7 @@ int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const voi
   if (c_lx) a.c_lx = make_double2(c_lx[0], c_lx[1]);
   if (c_z) a.c_z = make_double2(c_z[0], c_z[1]);
   if (c_acc) a.c_acc = make_double2(c_acc[0], c_acc[1]);
-  hipError_t e = lime::tiled_apply(a, static_cast<hipStream_t>(stream), lime::tiled_pick(d->batch, d->n),
-                                   is_fp64(d));
+  hipError_t e = lime::tiled_apply(a, static_cast<hipStream_t>(stream), tiled_tile(d), is_fp64(d));
   if (e != hipSuccess) return hip_fail(e, "lime_lindblad_apply launch");
   return LIME_OK;
 }

@@ -25,6 +25,72 @@
 
 namespace lime {
 
+namespace {
+
+// Copy a row-major n x n complex64 operator into an NP x P LDS buffer (zero padded).
+template <int NP, int P, int T>
+__device__ __forceinline__ void stage_operator(float2* dst, const float2* __restrict__ src, int n, int tid) {
+  for (int idx = tid; idx < NP * NP; idx += T) {
+    const int i = idx / NP, j = idx - (idx / NP) * NP;
+    float2 v = make_float2(0.f, 0.f);
+    if (i < n && j < n) v = src[(size_t)i * n + j];
+    dst[i * P + j] = v;
+  }
+}
+
+// Software-pipelined complex product over LDS operands for the wave's tiles:
+//   MODE 0: C += A B          (A rows from sA, B[k][j] = sB[k][j])
+//   MODE 2: C += i A M^dag    (A rows from sA, B[k][j] = conj(sM[j][k]), sM = sB)
+// sA / sB may be read up to 4 columns / rows past NP (spare LDS after each buffer).
+template <int NP, int P, int BPW, int MODE>
+__device__ __forceinline__ void lds_gemm(const float2* sA, const float2* sB, const int (&trow)[BPW],
+                                         const int (&tcol)[BPW], int lr, int lg, floatx4 (&cr)[BPW],
+                                         floatx4 (&ci)[BPW]) {
+  int oA[BPW], oB[BPW];
+#pragma unroll
+  for (int u = 0; u < BPW; ++u) {
+    oA[u] = (trow[u] + lr) * P + lg;
+    oB[u] = (MODE == 0) ? lg * P + tcol[u] + lr : (tcol[u] + lr) * P + lg;
+  }
+  float2 x[BPW], y[BPW];
+#pragma unroll
+  for (int u = 0; u < BPW; ++u) {
+    x[u] = sA[oA[u]];
+    y[u] = sB[oB[u]];
+  }
+#pragma unroll 2
+  for (int k0 = 0; k0 < NP; k0 += 4) {
+    float2 xn[BPW], yn[BPW];
+#pragma unroll
+    for (int u = 0; u < BPW; ++u) {
+      xn[u] = sA[oA[u] + k0 + 4];
+      yn[u] = sB[oB[u] + (MODE == 0 ? (k0 + 4) * P : k0 + 4)];
+    }
+#pragma unroll
+    for (int u = 0; u < BPW; ++u) {
+      const float2 a = x[u], h = y[u];
+      if (MODE == 0) {
+        cr[u] = mfma_f32(a.x, h.x, cr[u]);
+        ci[u] = mfma_f32(a.x, h.y, ci[u]);
+        cr[u] = mfma_f32(-a.y, h.y, cr[u]);
+        ci[u] = mfma_f32(a.y, h.x, ci[u]);
+      } else {
+        cr[u] = mfma_f32(a.x, h.y, cr[u]);
+        ci[u] = mfma_f32(a.x, h.x, ci[u]);
+        cr[u] = mfma_f32(-a.y, h.x, cr[u]);
+        ci[u] = mfma_f32(a.y, h.y, ci[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BPW; ++u) {
+      x[u] = xn[u];
+      y[u] = yn[u];
+    }
+  }
+}
+
+}  // namespace
+
 template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(MINW)))
 lindblad_rk4_resident(ResidentArgs a) {
@@ -37,9 +103,11 @@ lindblad_rk4_resident(ResidentArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   float2* sH = reinterpret_cast<float2*>(lds_raw);
   float2* sS = sH + NP * P;
-  // 4 spare rows after S absorb the (unused) fragment prefetch past the last k step
+  // 4 spare rows after S (and after sT, sL) absorb the unused fragment prefetch past the
+  // last k step
   float2* sT = sS + NP * P + 4 * P + 4;
-  double2* sRed = reinterpret_cast<double2*>(sT + (DENSE ? NP * P : 0));
+  float2* sL = sT + (DENSE ? NP * P + 4 * P + 4 : 0);
+  double2* sRed = reinterpret_cast<double2*>(sL + (DENSE ? NP * P + 4 * P + 4 : 0));
 
   const int b = blockIdx.x;
   if (b >= a.batch) return;
@@ -165,30 +233,20 @@ lindblad_rk4_resident(ResidentArgs a) {
           }
         }
       }
-      // C += i sum_k (L_k S) L_k^dag
+      // C += i sum_k (L_k S) R_k^dag, every factor staged through LDS (sL) so the fragment
+      // loops are the same pipelined LDS reads as the H_eff products.
       for (int kk = 0; DENSE && kk < a.n_dense; ++kk) {
         const float2* L = a.lops + ((size_t)set * a.n_dense + kk) * nn;
-        const float2* R = (a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + kk) * nn;
+        __syncthreads();                        // previous users of sL / sT are done
+        stage_operator<NP, P, T>(sL, L, n, tid);
+        __syncthreads();
         floatx4 tr[BPW], ti[BPW];
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
           tr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
           ti[u] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
-        for (int k0 = 0; k0 < NP; k0 += 4) {
-          const int kc = k0 + lg;
-#pragma unroll
-          for (int u = 0; u < BPW; ++u) {
-            const int ra = trow[u] + lr;
-            float2 x = make_float2(0.f, 0.f);
-            if (ra < n && kc < n) x = L[trow[u] * n + olr * n + k0 + olg];
-            const float2 y = sS[kc * P + tcol[u] + lr];
-            tr[u] = mfma_f32(x.x, y.x, tr[u]);
-            tr[u] = mfma_f32(-x.y, y.y, tr[u]);
-            ti[u] = mfma_f32(x.x, y.y, ti[u]);
-            ti[u] = mfma_f32(x.y, y.x, ti[u]);
-          }
-        }
+        lds_gemm<NP, P, BPW, 0>(sL, sS, trow, tcol, lr, lg, tr, ti);    // T = L S
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
           if (!tval[u]) continue;
@@ -196,22 +254,12 @@ lindblad_rk4_resident(ResidentArgs a) {
           for (int v = 0; v < 4; ++v)
             sT[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = make_float2(tr[u][v], ti[u][v]);
         }
-        __syncthreads();
-        for (int k0 = 0; k0 < NP; k0 += 4) {
-          const int kc = k0 + lg;
-#pragma unroll
-          for (int u = 0; u < BPW; ++u) {
-            const float2 x = sT[(trow[u] + lr) * P + kc];
-            const int jr = tcol[u] + lr;
-            float2 h = make_float2(0.f, 0.f);
-            if (jr < n && kc < n) h = R[tcol[u] * n + olr * n + k0 + olg];
-            cr[u] = mfma_f32(x.x, h.y, cr[u]);
-            cr[u] = mfma_f32(-x.y, h.x, cr[u]);
-            ci[u] = mfma_f32(x.x, h.x, ci[u]);
-            ci[u] = mfma_f32(x.y, h.y, ci[u]);
-          }
+        if (a.rops) {                           // general sandwich: R_k replaces L_k in sL
+          __syncthreads();
+          stage_operator<NP, P, T>(sL, a.rops + ((size_t)set * a.n_dense + kk) * nn, n, tid);
         }
         __syncthreads();
+        lds_gemm<NP, P, BPW, 2>(sT, sL, trow, tcol, lr, lg, cr, ci);    // C += i T R^dag
       }
       // k = -i C + D o S, formed in place: afterwards kr = ci, ki = -cr.
       if (has_d) {
@@ -369,10 +417,23 @@ lindblad_rk4_resident(ResidentArgs a) {
 // ------------------------------------------------------------------------ host launch
 namespace {
 
-// One launch shape per padded size NB: W waves, BPW tiles per wave (W * BPW >= NB^2),
-// chosen so that the FP64 state (16 doubles per tile per lane) fits the VGPR budget of
-// the occupancy the LDS footprint allows.
-constexpr int kWaves[7] = {0, 1, 2, 3, 8, 13, 12};
+// Launch shapes per padded size NB: W waves, BPW tiles per wave (W * BPW >= NB^2), MINW
+// the waves-per-SIMD the register budget is compiled for (512 / MINW VGPRs).  The FP64
+// state costs 16 doubles per tile per lane, so BPW sets the budget: 1 -> 6, 2 -> 4,
+// 3 -> 3, 4 -> 2.  Variant 0 is the default; the autotuner times the others.
+struct Shape {
+  int w, bpw;
+};
+constexpr int kMaxVariants = 4;
+constexpr Shape kShapes[7][kMaxVariants] = {
+    {},
+    {{1, 1}, {0, 0}, {0, 0}, {0, 0}},
+    {{2, 2}, {4, 1}, {1, 4}, {0, 0}},
+    {{3, 3}, {9, 1}, {5, 2}, {0, 0}},
+    {{8, 2}, {4, 4}, {16, 1}, {6, 3}},
+    {{13, 2}, {9, 3}, {7, 4}, {0, 0}},
+    {{12, 3}, {0, 0}, {0, 0}, {0, 0}},
+};
 
 template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC>
 hipError_t launch_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
@@ -391,9 +452,11 @@ hipError_t launch_rec(const ResidentArgs& a, size_t lds, hipStream_t stream) {
              : launch_one<NB, W, BPW, MINW, DENSE, false>(a, lds, stream);
 }
 
-template <int NB, int W, int BPW, int MINW>
+template <int NB, int W, int BPW>
 hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream) {
-  if constexpr (NB <= 5) {
+  static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
+  constexpr int MINW = BPW == 1 ? 6 : BPW == 2 ? 4 : BPW == 3 ? 3 : 2;
+  if constexpr (NB <= 4) {
     if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true>(a, lds, stream);
   } else {
     if (a.n_dense > 0) return hipErrorInvalidValue;
@@ -403,24 +466,44 @@ hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream) {
 
 }  // namespace
 
-int resident_nb_max(int n_dense) { return n_dense > 0 ? 5 : 6; }
+int resident_nb_max(int n_dense) { return n_dense > 0 ? 4 : 6; }
 
-size_t resident_lds_bytes(int nb, int n_dense, int n_eops) {
+int resident_variants(int nb) {
+  if (nb < 1 || nb > 6) return 0;
+  int c = 0;
+  while (c < kMaxVariants && kShapes[nb][c].w > 0) ++c;
+  return c;
+}
+
+int resident_waves(int nb, int variant) { return kShapes[nb][variant].w; }
+
+size_t resident_lds_bytes(int nb, int variant, int n_dense, int n_eops) {
   const int np = nb * 16, p = np + 1;
-  size_t bytes = ((size_t)(n_dense > 0 ? 3 : 2) * np * p + 4 * p + 4) * sizeof(float2);
-  bytes += (size_t)(1 + n_eops) * kWaves[nb] * sizeof(double2);
+  const size_t buf = (size_t)np * p + 4 * p + 4;   // one operator buffer + prefetch spare rows
+  size_t bytes = (np * p + (n_dense > 0 ? 3 : 1) * buf) * sizeof(float2);
+  bytes += (size_t)(1 + n_eops) * kShapes[nb][variant].w * sizeof(double2);
   return bytes;
 }
 
-hipError_t resident_launch(const ResidentArgs& a, int nb, hipStream_t stream) {
-  const size_t lds = resident_lds_bytes(nb, a.n_dense, a.n_eops);
-  switch (nb) {
-    case 1: return launch_shape<1, 1, 1, 4>(a, lds, stream);
-    case 2: return launch_shape<2, 2, 2, 4>(a, lds, stream);
-    case 3: return launch_shape<3, 3, 3, 3>(a, lds, stream);
-    case 4: return launch_shape<4, 8, 2, 4>(a, lds, stream);
-    case 5: return launch_shape<5, 13, 2, 4>(a, lds, stream);
-    case 6: return launch_shape<6, 12, 3, 3>(a, lds, stream);
+hipError_t resident_launch(const ResidentArgs& a, int nb, int variant, hipStream_t stream) {
+  if (variant < 0 || variant >= resident_variants(nb)) return hipErrorInvalidValue;
+  const size_t lds = resident_lds_bytes(nb, variant, a.n_dense, a.n_eops);
+  switch (nb * 10 + variant) {
+    case 10: return launch_shape<1, 1, 1>(a, lds, stream);
+    case 20: return launch_shape<2, 2, 2>(a, lds, stream);
+    case 21: return launch_shape<2, 4, 1>(a, lds, stream);
+    case 22: return launch_shape<2, 1, 4>(a, lds, stream);
+    case 30: return launch_shape<3, 3, 3>(a, lds, stream);
+    case 31: return launch_shape<3, 9, 1>(a, lds, stream);
+    case 32: return launch_shape<3, 5, 2>(a, lds, stream);
+    case 40: return launch_shape<4, 8, 2>(a, lds, stream);
+    case 41: return launch_shape<4, 4, 4>(a, lds, stream);
+    case 42: return launch_shape<4, 16, 1>(a, lds, stream);
+    case 43: return launch_shape<4, 6, 3>(a, lds, stream);
+    case 50: return launch_shape<5, 13, 2>(a, lds, stream);
+    case 51: return launch_shape<5, 9, 3>(a, lds, stream);
+    case 52: return launch_shape<5, 7, 4>(a, lds, stream);
+    case 60: return launch_shape<6, 12, 3>(a, lds, stream);
     default: return hipErrorInvalidValue;
   }
 }

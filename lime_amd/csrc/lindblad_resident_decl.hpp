@@ -21,7 +21,9 @@ struct ResidentArgs {
 };
 
 int resident_nb_max(int n_dense);
-size_t resident_lds_bytes(int nb, int n_dense, int n_eops);
-hipError_t resident_launch(const ResidentArgs& a, int nb, hipStream_t stream);
+int resident_variants(int nb);
+int resident_waves(int nb, int variant);
+size_t resident_lds_bytes(int nb, int variant, int n_dense, int n_eops);
+hipError_t resident_launch(const ResidentArgs& a, int nb, int variant, hipStream_t stream);
 
 }  // namespace lime

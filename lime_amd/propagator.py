@@ -98,13 +98,14 @@ class LindbladPropagator:
     """
 
     def __init__(self, H, c_ops=None, group=1, precision="fp32", sandwich=None,
-                 force_tiled=False):
+                 force_tiled=False, variant=0):
         """``sandwich``: optional explicit (L, R) lists of (S,N,N) operators contributing
         sum_k L_k X R_k^dag (used instead of folding dense jump operators; H must then
         already be the effective generator).  ``force_tiled`` selects the per-stage tiled
         kernels even for n <= 96."""
         self.dev = device()
         self.force_tiled = bool(force_tiled)
+        self.variant = int(variant)
         self._work = None
         if precision not in _PREC_FLAGS:
             raise ValueError(f"unknown precision {precision!r}; expected one of {sorted(_PREC_FLAGS)}")
@@ -142,6 +143,7 @@ class LindbladPropagator:
         self.dev = device()
         self.precision = precision
         self.force_tiled = bool(force_tiled)
+        self.variant = 0
         self._work = None
         self._lib = _lib.load()
         cdt = torch.complex128 if precision == "fp64" else torch.complex64
@@ -180,6 +182,7 @@ class LindbladPropagator:
         d.batch, d.n, d.group, d.n_dense = B, self.n, self.group, self.n_dense
         d.eop_group = B
         d.flags = _PREC_FLAGS[self.precision]
+        d.variant = self.variant
         d.heff = self.heff.data_ptr()
         d.dmat = self.dmat.data_ptr() if self.dmat is not None else None
         d.lops = self.lops.data_ptr() if self.lops is not None else None
@@ -214,6 +217,13 @@ class LindbladPropagator:
             ctypes.byref(d), x64.data_ptr(), ptr(z), ptr(y), ptr(y64), ptr(acc),
             pair(c_lx), pair(c_z), pair(c_acc), ctypes.c_void_p(s.cuda_stream)))
 
+    def variants(self, n_eops=0):
+        """Number of launch-shape variants the library offers for this batch."""
+        d = self._descriptor(self.batch)
+        d.n_eops = n_eops
+        d.flags |= (_lib.LIME_F_FORCE_TILED if self.force_tiled else 0)
+        return int(self._lib.lime_lindblad_variants(ctypes.byref(d)))
+
     def max_n(self):
         return self._lib.lime_lindblad_max_n(self.n_dense, _PREC_FLAGS[self.precision])
 
@@ -231,6 +241,7 @@ class LindbladPropagator:
         d.eop_group = int(eop_group or B)
         d.flags = _PREC_FLAGS[self.precision] | (_lib.LIME_F_RENORM_TRACE if renorm else 0) \
             | (_lib.LIME_F_FORCE_TILED if self.force_tiled else 0)
+        d.variant = self.variant
         d.heff = self.heff.data_ptr()
         d.dmat = self.dmat.data_ptr() if self.dmat is not None else None
         d.lops = self.lops.data_ptr() if self.lops is not None else None

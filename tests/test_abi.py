@@ -37,7 +37,7 @@ def test_every_declared_symbol_is_exported(lib):
 def test_version_and_limits(lib):
     assert b"gfx950" in lib.lime_version()
     assert lib.lime_lindblad_max_n(0, _lib.LIME_F_PREC_FP32) == 96
-    assert lib.lime_lindblad_max_n(3, _lib.LIME_F_PREC_FP32) == 80
+    assert lib.lime_lindblad_max_n(3, _lib.LIME_F_PREC_FP32) == 64
     d = _lib.LindbladBatch()
     d.batch, d.n, d.group, d.eop_group, d.flags = 2, 20, 1, 1, _lib.LIME_F_PREC_FP64
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 2 * 400 * 16 * 3

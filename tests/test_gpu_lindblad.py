@@ -230,3 +230,29 @@ def test_fp64_tiled_variant_matches_oracle(n, kd):
     print(f"fp64 n={n} kd={kd} err {err:.3e}")
     assert err < 1e-13
     assert np.max(np.abs(obs[0].cpu().numpy() - ref_obs)) < 1e-12
+
+
+@pytest.mark.parametrize("n,kd", [(8, 1), (30, 2), (48, 0), (64, 2), (80, 0), (96, 0)])
+def test_every_launch_variant_matches_oracle(n, kd):
+    """Each resident launch shape (and the tiled tile choices) gives the same physics."""
+    rng = np.random.default_rng(900 + n)
+    H = rand_herm(rng, n)
+    c_ops = [0.1 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(kd)]
+    c_ops += [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_dm(rng, n), rand_c(rng, n, n)])
+    A = [rand_herm(rng, n)]
+    dt, nt = 0.02, 20
+    refs = [ol.propagate(H, X0[b], c_ops, dt, nt, A, nout=5) for b in range(2)]
+    prop = _prop()(H, c_ops, group=2)
+    nv = prop.variants(n_eops=1)
+    assert nv >= 1
+    for v in range(nv):
+        prop.variant = v
+        rho, obs, _ = prop.run(X0, dt, nt, e_ops=A, nout=5)
+        for b in range(2):
+            ref, ref_obs, _ = refs[b]
+            scale = max(1.0, np.max(np.abs(ref)))
+            err = np.max(np.abs(rho[b].cpu().numpy() - ref)) / scale
+            print(f"variant n={n} kd={kd} v={v} b={b} rel err {err:.3e}")
+            assert err < 2e-7
+            assert np.max(np.abs(obs[b].cpu().numpy() - ref_obs)) < 1e-6 * max(1.0, np.max(np.abs(ref_obs)))
