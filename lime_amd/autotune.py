@@ -1,0 +1,96 @@
+"""Launch-shape autotuner for the Lindblad kernels.
+
+The library exposes, per descriptor, a small set of launch-shape variants
+(``lime_lindblad_variants``): for the resident kernel the (waves, tiles-per-wave) shapes of
+each padded size, for the tiled kernels the 32x32 or 64x64 output tile.  ``tune`` times
+every variant on the caller's own batch for a few RK4 steps (after one warm-up launch,
+HIP events on the launch stream), keeps the fastest, and caches the choice per
+(n, dense operators, precision, batch bucket, observables) in a JSON file
+(``LIME_TUNE_CACHE`` or ``lime_amd/_tune_cache.json``).
+"""
+import json
+import os
+import threading
+
+import torch
+
+_lock = threading.Lock()
+_CACHE_PATH = os.environ.get("LIME_TUNE_CACHE",
+                             os.path.join(os.path.dirname(os.path.abspath(__file__)), "_tune_cache.json"))
+_cache = None
+
+
+def _load():
+    global _cache
+    if _cache is None:
+        try:
+            with open(_CACHE_PATH) as f:
+                _cache = json.load(f)
+        except (OSError, ValueError):
+            _cache = {}
+    return _cache
+
+
+def _save():
+    try:
+        tmp = _CACHE_PATH + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(_cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, _CACHE_PATH)
+    except OSError:
+        pass
+
+
+def _bucket(batch):
+    b = 1
+    while b < batch:
+        b *= 2
+    return b
+
+
+def key_of(prop, n_eops=0):
+    return (f"n{prop.n}:kd{prop.n_dense}:{prop.precision}:b{_bucket(prop.batch)}"
+            f":e{int(n_eops > 0)}:t{int(prop.force_tiled)}")
+
+
+def tune(prop, rho, dt, steps=4, n_eops=0, force=False, verbose=False):
+    """Pick the fastest launch variant for ``prop`` on operators shaped like ``rho``.
+
+    ``rho`` is not modified (a scratch copy is advanced).  Sets ``prop.variant`` and
+    returns (variant, {variant: ms}).
+    """
+    with _lock:
+        cache = _load()
+        k = key_of(prop, n_eops)
+        if not force and k in cache:
+            prop.variant = int(cache[k]["variant"])
+            return prop.variant, cache[k].get("ms", {})
+    nv = prop.variants(n_eops=n_eops)
+    if nv <= 1:
+        prop.variant = 0
+        return 0, {}
+    scratch = rho.clone()
+    times = {}
+    stream = torch.cuda.current_stream(prop.dev)
+    for v in range(nv):
+        prop.variant = v
+        try:
+            prop.step(scratch, dt, 1)                  # warm-up (and attribute set-up)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            prop.step(scratch, dt, steps)
+            e1.record(stream)
+            torch.cuda.synchronize(prop.dev)
+            times[v] = e0.elapsed_time(e1) / steps
+        except RuntimeError as err:                    # a variant this size cannot launch
+            if verbose:
+                print(f"[tune] variant {v} skipped: {err}")
+    best = min(times, key=times.get)
+    prop.variant = best
+    if verbose:
+        print(f"[tune] {key_of(prop, n_eops)}: " + ", ".join(f"v{v}={t:.3f}ms" for v, t in sorted(times.items()))
+              + f" -> v{best}")
+    with _lock:
+        _load()[key_of(prop, n_eops)] = {"variant": best, "ms": {str(v): t for v, t in times.items()}}
+        _save()
+    return best, times

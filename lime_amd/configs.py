@@ -115,14 +115,16 @@ def random_pure_dm(rng, n):
 
 def ragged_gue(seed=3, count=2048, sizes=tuple(range(16, 257, 16)), gamma=0.01, n_dense=4,
                fro=0.1):
+    """The n on-site dephasing operators are given compactly (``dephasing`` = gamma); the
+    dense jump operators as matrices in ``c_ops``."""
     rng = np.random.default_rng(seed)
     ns = rng.choice(np.asarray(sizes), size=count)
     inst = []
     for n in ns:
         n = int(n)
         H = gue(rng, n)
-        c_ops = onsite_dephasing(n, gamma, range(n)) + [dense_jump(rng, n, fro) for _ in range(n_dense)]
-        inst.append(dict(H=H, c_ops=c_ops, rho0=random_pure_dm(rng, n)))
+        c_ops = [dense_jump(rng, n, fro) for _ in range(n_dense)]
+        inst.append(dict(H=H, c_ops=c_ops, dephasing=gamma, rho0=random_pure_dm(rng, n)))
     return dict(instances=inst, sizes=ns, dt=0.02, nt=1000, seed=seed)
 
 
