@@ -25,6 +25,8 @@ import time
 import numpy as np
 
 PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_BF16_MFMA_TFLOPS = 2500.0
+PEAK_FP64_MFMA_TFLOPS = 78.6
 
 
 def log(*a):
@@ -81,6 +83,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU comparator")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy checks")
+    ap.add_argument("--workload", default="lattice65",
+                    choices=["lattice65", "ragged", "gradient", "dense512"],
+                    help="lattice65 (default): 4096 x 3 QRT operators at n=65; ragged: 2048 "
+                         "instances n in {16..256} in one graph; gradient: adjoint of the "
+                         "128-site chain spectrum loss; dense512: n=512 with 16 dense L_k")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "3xbf16", "fp64"],
+                    help="GEMM arithmetic (ragged / dense512)")
+    ap.add_argument("--batch", type=int, default=32, help="dense512: operators per launch")
+    ap.add_argument("--instances", type=int, default=2048, help="ragged: number of instances")
+    ap.add_argument("--check-steps", type=int, default=100,
+                    help="dense512: RK4 steps of the FP64 GPU reference comparison")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -90,11 +103,19 @@ def main():
         sys.exit(2)
 
     import torch
+    torch.cuda.set_device(torch.device("cuda", 0))
+    if args.workload != "lattice65":
+        rk_default = {"ragged": 10, "dense512": 10, "gradient": 0}[args.workload]
+        if args.rk_per_step == ap.get_default("rk_per_step"):
+            args.rk_per_step = rk_default
+        result = {"ragged": bench_ragged, "gradient": bench_gradient,
+                  "dense512": bench_dense512}[args.workload](args)
+        print(json.dumps(result), flush=True)
+        return
     from lime_amd import configs
     from lime_amd.propagator import LindbladPropagator
 
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     cfg = configs.lattice_spectrum(seed=args.seed, n_real=args.realisations)
     R, n = args.realisations, cfg["n"]
     B = 3 * R
@@ -238,6 +259,266 @@ def plain_fp64(cfg, B, dt, value, steps=6):
     log(f"[plain] FP64 GPU propagator: {pv:.4g} rho-steps/s ({ms / steps:.2f} ms per RK4 step)")
     return {"plain_fp64_gpu": {"value": pv, "unit": "rho-steps/s", "rk_steps_timed": steps},
             "speedup_vs_plain_fp64": value / pv}
+
+
+# ----------------------------------------------------------------------------------------
+# Further workloads (not the default line): uneven batch, adjoint gradient, n = 512 dense.
+
+def _base(metric, value, args, ms_per_step, dtype, data, config, roofline):
+    return {"metric": metric, "value": value, "unit": "rho-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": data, "config": config, "roofline": roofline}
+
+
+def _roofline(kernel, tflops, launch_ms, precision):
+    r = {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+         "frac": tflops / PEAK_FP32_MFMA_TFLOPS, "traffic": None, "kernel": kernel,
+         "launch_ms_mean": launch_ms}
+    if precision == "3xbf16":
+        # each complex multiply-add issues 12 BF16 slot products (24 flops) for the canonical 8
+        r["issued_bf16_tflops"] = 3.0 * tflops
+        r["issued_bf16_frac"] = 3.0 * tflops / PEAK_BF16_MFMA_TFLOPS
+    if precision == "fp64":
+        r["peak"] = PEAK_FP64_MFMA_TFLOPS
+        r["frac"] = tflops / PEAK_FP64_MFMA_TFLOPS
+    return r
+
+
+def _dtype(precision):
+    return {"fp32": "c64-fp32mfma+f64", "3xbf16": "c64-3xbf16mfma+f64", "fp64": "c128-fp64mfma"}[precision]
+
+
+def _cpu_sample(H, c_ops, X, dt, budget_s, label):
+    """lime's operation sequence on one operator (NumPy complex128, all BLAS threads)."""
+    from oracle import lindblad as ol
+    try:
+        from threadpoolctl import threadpool_info
+        blas = (threadpool_info() or [{}])[0]
+    except ImportError:                       # pragma: no cover
+        blas = {}
+    ol.rk4(X.copy(), ol.liouvillian_ref, dt, H, c_ops)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        X = ol.rk4(X, ol.liouvillian_ref, dt, H, c_ops)
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": steps / el, "unit": "rho-steps/s", "cores": int(blas.get("num_threads", 1)),
+            "kind": "port", "sample": f"{label}: {steps} RK4 steps of lime's 2+5K operation "
+                                      f"sequence, NumPy complex128, {el:.1f} s; BLAS "
+                                      f"{blas.get('internal_api', '?')}"}
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    for i in range(warmup):
+        fn(i)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
+        fn(warmup + i)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    return ev[0].elapsed_time(ev[-1]), per
+
+
+def bench_dense512(args):
+    """n = 512 GUE (spectral radius 1), 16 dense L_k (||L_k||_F = 0.05),
+    rho0 = (1 - eta) I/n + eta psi psi^dag, dt = 0.01; batch 1 or 32."""
+    import torch
+    from lime_amd import configs
+    from lime_amd.propagator import LindbladPropagator
+    B, prec, chunk = args.batch, args.precision, args.rk_per_step
+    distinct = min(B, 4)
+    cfg = configs.dense_single(batch=distinct)
+    reps = -(-B // distinct)
+    H = np.concatenate([cfg["H"]] * reps)[:B]
+    Ls = [np.concatenate([L] * reps)[:B] for L in cfg["c_ops"]]
+    n, kd = 512, len(Ls)
+    log(f"[dense512] batch {B} ({distinct} distinct systems), precision {prec}, {chunk} RK4 steps per launch")
+    prop = LindbladPropagator(H, Ls, precision=prec)
+    from lime_amd import autotune
+    rho0 = torch.as_tensor(np.broadcast_to(cfg["rho0"], (B, n, n)).copy(), device="cuda")
+    autotune.tune(prop, rho0, cfg["dt"], steps=2, verbose=True)
+    X = rho0.clone()
+    total_ms, per = _timed(lambda i: prop.step(X, cfg["dt"], chunk), args.steps, args.warmup)
+    rho_steps = B * chunk * args.steps
+    value = rho_steps / (total_ms / 1e3)
+    fl = flops_per_rho_step(n, kd)
+    tflops = fl * B * chunk / (np.mean(per) / 1e3) / 1e12
+    log(f"[dense512] {value:.4g} rho-steps/s, {tflops:.1f} TFLOP/s canonical")
+    out = _base(f"RK4 rho-steps/s (n=512, 16 dense L_k, {prec})", value, args, total_ms / args.steps,
+                _dtype(prec), "synthetic (seeded GUE + dense jump operators)",
+                {"workload": f"dense_single: n=512, 16 dense L_k, batch {B}, dt=0.01",
+                 "batch": B, "n": n, "n_dense": kd, "eta": cfg["eta"], "seed": cfg["seed"],
+                 "rk_steps_per_launch": chunk, "variant": prop.variant,
+                 "flops_per_rho_step": fl},
+                _roofline("lindblad_stage_tiled + lindblad_left_tiled", tflops, float(np.mean(per)) / (4 * chunk), prec))
+    out["roofline"]["launch_ms_mean"] = None
+    out["roofline"]["rk_step_ms_mean"] = float(np.mean(per)) / chunk
+    if not args.no_check and prec != "fp64":
+        m = min(B, 2)
+        ns = args.check_steps
+        ref = LindbladPropagator(H[:m], [L[:m] for L in Ls], precision="fp64")
+        got = LindbladPropagator(H[:m], [L[:m] for L in Ls], precision=prec, variant=prop.variant)
+        Xr, Xg = rho0[:m].clone(), rho0[:m].clone()
+        ref.step(Xr, cfg["dt"], ns)
+        got.step(Xg, cfg["dt"], ns)
+        torch.cuda.synchronize()
+        d = (Xg - Xr)
+        pops = torch.diagonal(d, dim1=-2, dim2=-1).abs().max().item()
+        out["check_vs_fp64_gpu"] = {
+            "rk_steps": ns, "operators": m,
+            "pop_max_abs_err": pops,
+            "max_abs_err": d.abs().max().item(),
+            "max_rel_err": d.abs().max().item() / Xr.abs().max().item(),
+            "trace_residual": (torch.diagonal(Xg, dim1=-2, dim2=-1).sum(-1) - 1).abs().max().item(),
+            "hermiticity_residual": (Xg - Xg.conj().transpose(-1, -2)).abs().max().item()}
+        log(f"[dense512] vs FP64 GPU reference over {ns} steps: {out['check_vs_fp64_gpu']}")
+    if not args.no_plain:
+        from lime_amd.plain import PlainFP64Propagator
+        p = PlainFP64Propagator(H[:min(B, 4)], [L[:min(B, 4)] for L in Ls])
+        Y = rho0[:min(B, 4)].clone()
+        Y = p.step(Y, cfg["dt"])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(2):
+            Y = p.step(Y, cfg["dt"])
+        e1.record()
+        torch.cuda.synchronize()
+        pv = min(B, 4) * 2 / (e0.elapsed_time(e1) / 1e3)
+        out["plain_fp64_gpu"] = {"value": pv, "unit": "rho-steps/s", "batch": min(B, 4)}
+        out["speedup_vs_plain_fp64"] = value / pv
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_sample(cfg["H"][0], [L[0] for L in cfg["c_ops"]], cfg["rho0"].copy(),
+                                          cfg["dt"], 10.0, "system 0 (n=512, K=16 dense)")
+    return out
+
+
+def bench_ragged(args):
+    """2048 instances, n uniform in {16, 32, ..., 256}: GUE H (spectral radius 1), n on-site
+    dephasing operators (gamma = 0.01) + 4 dense L_k (||L_k||_F = 0.1), rho0 = psi psi^dag,
+    dt = 0.02; all size classes replayed from one captured graph."""
+    import torch
+    from lime_amd import configs
+    from lime_amd.ragged import RaggedLindblad
+    t0 = time.perf_counter()
+    cfg = configs.ragged_gue(count=args.instances)
+    log(f"[ragged] {args.instances} instances generated in {time.perf_counter() - t0:.1f} s")
+    rb = RaggedLindblad(cfg["instances"], precision=args.precision, tune=True, verbose=True)
+    chunk, dt = args.rk_per_step, cfg["dt"]
+    rb.capture(dt, chunk)
+    total_ms, per = _timed(lambda i: rb.graph.replay(), args.steps, args.warmup)
+    count = len(cfg["instances"])
+    value = count * chunk * args.steps / (total_ms / 1e3)
+    tflops = rb.flops_per_step * chunk / (np.mean(per) / 1e3) / 1e12
+    log(f"[ragged] {value:.4g} rho-steps/s, {tflops:.1f} TFLOP/s canonical")
+    sizes = np.asarray(cfg["sizes"])
+    out = _base(f"RK4 rho-steps/s (uneven batch n in 16..256, {args.precision})", value, args,
+                total_ms / args.steps, _dtype(args.precision),
+                "synthetic (seeded GUE instances, on-site dephasing + 4 dense L_k)",
+                {"workload": f"ragged_gue: {count} instances, n uniform in 16..256 step 16, dt=0.02, "
+                             f"one hipGraph replay = {chunk} RK4 steps of every instance",
+                 "instances": count, "size_classes": len(rb.classes),
+                 "mean_n3": float(np.mean(sizes.astype(np.float64) ** 3)),
+                 "rk_steps_per_replay": chunk, "seed": cfg["seed"],
+                 "flops_per_batch_step": rb.flops_per_step,
+                 "variants": {str(c["n"]): c["prop"].variant for c in rb.classes}},
+                _roofline("hipGraph of resident + tiled launches", tflops, float(np.mean(per)), args.precision))
+    if not args.no_check:
+        from oracle import lindblad as ol
+        done = (args.warmup + args.steps) * chunk
+        states = rb.states()
+        errs = {}
+        for target in (16, 128, 256):
+            idx = [i for i, n in enumerate(sizes) if n == target]
+            if not idx:
+                continue
+            i = idx[0]
+            inst = cfg["instances"][i]
+            n = target
+            c_ops = list(inst["c_ops"]) + [np.sqrt(inst["dephasing"]) * np.diag(np.eye(n)[k]) for k in range(n)]
+            ref, _, _ = ol.propagate(inst["H"], inst["rho0"], c_ops, dt, done)
+            g = states[i]
+            errs[str(n)] = {"max_abs_err": float(np.max(np.abs(g - ref))),
+                            "pop_max_abs_err": float(np.max(np.abs(np.diag(g) - np.diag(ref)))),
+                            "trace_residual": float(abs(np.trace(g) - 1))}
+        out["check_vs_oracle"] = {"rk_steps": done, "by_n": errs}
+        log(f"[ragged] vs oracle after {done} steps: {errs}")
+    if not args.no_cpu:
+        i = int(np.argmin(np.abs(sizes - 128)))
+        inst = cfg["instances"][i]
+        n = int(sizes[i])
+        c_ops = list(inst["c_ops"]) + [np.sqrt(inst["dephasing"]) * np.diag(np.eye(n)[k]) for k in range(n)]
+        out["cpu_baseline"] = _cpu_sample(inst["H"], c_ops, inst["rho0"].copy(), dt, 10.0,
+                                          f"instance {i} (n={n}, K={len(c_ops)})")
+    return out
+
+
+def bench_gradient(args):
+    """256 chains of 128 sites (+ ground state, n = 129), shared gamma = 0.02, dt = 0.05,
+    2000 RK4 steps; spectrum L2 loss against a target from a second seeded parameter set;
+    gradients w.r.t. eps, J and gamma by the adjoint kernels (checkpoints every 64 steps).
+    One bench step = forward + backward of the whole batch."""
+    import torch
+    from lime_amd import configs
+    from lime_amd.adjoint import ChainAbsorption, spectrum_loss
+    cfg = configs.chain_gradient()
+    tgt = configs.chain_gradient(seed=cfg["seed"] + 1000)
+    dev = torch.device("cuda")
+    B, ns, nt, K, dt = cfg["eps"].shape[0], cfg["eps"].shape[1], cfg["nt"], cfg["checkpoint"], cfg["dt"]
+    mu = torch.tensor(cfg["mu"][:, 0], device=dev)
+    prec = args.precision
+    with torch.no_grad():
+        target = ChainAbsorption.apply(torch.tensor(tgt["eps"], device=dev), torch.tensor(tgt["J"], device=dev),
+                                       torch.tensor(cfg["gamma"], dtype=torch.float64, device=dev), mu, dt, nt, K, prec)
+    torch.cuda.synchronize()
+    times = []
+    for i in range(args.warmup + args.steps):
+        eps = torch.tensor(cfg["eps"], device=dev, requires_grad=True)
+        J = torch.tensor(cfg["J"], device=dev, requires_grad=True)
+        gamma = torch.tensor(cfg["gamma"], dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        C = ChainAbsorption.apply(eps, J, gamma, mu, dt, nt, K, prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = spectrum_loss(C, target, dt).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        log(f"[gradient] pass {i}: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s, loss {loss.item():.6e}")
+        if i >= args.warmup:
+            times.append((t1 - t0, t2 - t1))
+    fwd = float(np.mean([t[0] for t in times]))
+    bwd = float(np.mean([t[1] for t in times]))
+    value = B * nt / (fwd + bwd)
+    n = ns + 1
+    out = _base(f"RK4 rho-steps/s of forward + adjoint gradient (n=129 chain, {prec})", value, args,
+                1e3 * (fwd + bwd), _dtype(prec), "synthetic (seeded 128-site chain ensemble)",
+                {"workload": f"chain_gradient: {B} chains x 128 sites (+ground, n=129), dt=0.05, "
+                             f"{nt} RK4 steps, checkpoints every {K} steps; one step = forward + backward",
+                 "batch": B, "n": n, "rk_steps": nt, "checkpoint": K, "seed": cfg["seed"]},
+                _roofline("forward: lindblad_stage_tiled; backward: lindblad_apply_tiled + adjoint_contract",
+                          flops_per_rho_step(n) * B * nt / fwd / 1e12, None, prec))
+    out["forward_s"] = fwd
+    out["backward_s"] = bwd
+    out["forward_rho_steps_per_s"] = B * nt / fwd
+    out["grad_norms"] = {"eps": float(eps.grad.norm()), "J": float(J.grad.norm()), "gamma": float(gamma.grad)}
+    if not args.no_cpu:
+        from lime_amd.configs import onsite_dephasing
+        Hc = np.zeros((n, n), dtype=np.complex128)
+        Hc[1:, 1:] = np.diag(cfg["eps"][0]) + np.diag(cfg["J"][0], 1) + np.diag(cfg["J"][0], -1)
+        c_ops = onsite_dephasing(n, cfg["gamma"], range(1, n))
+        X = np.zeros((n, n), dtype=np.complex128)
+        X[1:, 0] = cfg["mu"][:, 0]
+        out["cpu_baseline"] = _cpu_sample(Hc, c_ops, X, dt, 10.0, "chain 0 forward (n=129, K=128 on-site)")
+    return out
 
 
 if __name__ == "__main__":

@@ -34,7 +34,9 @@ extern "C" {
                                       matrices only; never set for QRT operators)       */
 #define LIME_F_PREC_MASK 0x30      /* arithmetic of the right-hand-side GEMMs:          */
 #define LIME_F_PREC_FP32 0x00      /*   complex64 operands on FP32 MFMA (default)       */
-#define LIME_F_PREC_3XBF16 0x10    /*   3-term BF16 split products on BF16 MFMA         */
+#define LIME_F_PREC_3XBF16 0x10    /*   complex64 operands split x = hi + lo (BF16) and  */
+                                   /*   multiplied as hi*hi + hi*lo + lo*hi on BF16 MFMA */
+                                   /*   (tiled kernels; lime_lindblad_rk4 only)          */
 #define LIME_F_PREC_FP64 0x20      /*   complex128 operands on FP64 MFMA (tiled kernels;
                                         operator sets and x/y64 of lime_lindblad_apply are
                                         then complex128)                                  */

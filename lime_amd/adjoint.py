@@ -8,8 +8,9 @@ Model (Hilbert space = ground state |g> = index 0 + n_sites sites):
     H = sum_j eps_j |j><j| + J_j (|j><j+1| + h.c.),  L_j = sqrt(gamma) |j><j|  (sites only),
     mu = sum_j mu_j (|j><g| + |g><j|),  X_0 = mu |g><g|,  C_k = Tr[mu X_k],  k = 1..Nt.
 
-Forward: the fused kernel (resident for n <= 96, tiled above) records C_k every step and
-the checkpoints X_{cK} every K steps (default K = 64).
+Forward: the fused kernel (resident for n <= 96, tiled above; launch shape from the
+autotuner) records C_k every step, K steps per launch; the state after each launch is the
+checkpoint X_{cK} (default K = 64).
 
 Backward, for a real loss with torch's gradient G_k = dL/dC_k^* convention
 (dL = Re sum_k conj(G_k) dC_k): the source of step k is g_k = G_k mu^dag, and with the RK4
@@ -61,7 +62,7 @@ class ChainQRT:
     """Forward propagation with checkpoints and the adjoint backward pass."""
 
     def __init__(self, eps, J, gamma, mu_sites, dt, nt, checkpoint=64, force_tiled=False,
-                 precision="fp32"):
+                 precision="fp32", tune=True):
         self.dev = device()
         self.eps = eps.to(self.dev, torch.float64)
         self.J = J.to(self.dev, torch.float64)
@@ -72,9 +73,12 @@ class ChainQRT:
         heff, dmat = chain_generator(self.eps, self.J, torch.as_tensor(gamma, device=self.dev))
         self.prop = LindbladPropagator.from_folded(heff, dmat, force_tiled=force_tiled,
                                                    precision=precision)
+        self.x0 = torch.zeros((self.B, self.n, self.n), dtype=torch.complex128, device=self.dev)
+        if tune:
+            from . import autotune
+            autotune.tune(self.prop, self.x0, self.dt)
         self.adj = self.prop.adjoint()
         self.mu = dipole_operator(torch.as_tensor(mu_sites, device=self.dev), self.n)
-        self.x0 = torch.zeros((self.B, self.n, self.n), dtype=torch.complex128, device=self.dev)
         self.x0[:, :, 0] = self.mu[:, 0]
 
     def forward(self):
@@ -85,19 +89,16 @@ class ChainQRT:
         obs = torch.zeros((B, self.nt, 1), dtype=torch.complex128, device=self.dev)
         ncp = self.nt // self.K
         cps = torch.empty((B, ncp, n, n), dtype=torch.complex128, device=self.dev) if ncp else None
-        self.x_start = X.clone()
+        # one pass: K steps per launch (C(t) recorded every step), the state after each
+        # launch is the next checkpoint
         done = 0
         while done < self.nt:
-            m = min(self.K * 16, self.nt - done)
+            m = min(self.K, self.nt - done)
             self.prop.step(X, self.dt, m, eops_t=E_t, eop_group=B, obs=obs, nout=1,
                            step_offset=done, rec_capacity=self.nt)
             done += m
-        # checkpoints: a second pass recording states every K steps (cheap next to the
-        # 2.75x backward; keeps the C(t) pass free of state traffic)
-        if ncp:
-            X2 = self.x0.clone()
-            self.prop.step(X2, self.dt, ncp * self.K, states=cps, nout=self.K, step_offset=0,
-                           rec_capacity=ncp)
+            if m == self.K and done // self.K <= ncp:
+                cps[:, done // self.K - 1].copy_(X)
         self.checkpoints = cps
         return obs[:, :, 0]
 

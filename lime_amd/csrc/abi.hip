@@ -31,9 +31,14 @@ int resident_variant(const lime_lindblad_batch* d) {
   return (d->variant > 0 && d->variant < lime::resident_variants(nb)) ? d->variant : 0;
 }
 
+bool is_split_bf16(const lime_lindblad_batch* d) {
+  return (d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_3XBF16;
+}
+
 bool use_resident(const lime_lindblad_batch* d) {
   if (d->flags & LIME_F_FORCE_TILED) return false;
-  if (is_fp64(d)) return false;             // the FP64 variant is built for the tiled kernels
+  // the FP64 and 3xBF16 variants are built for the tiled kernels
+  if ((d->flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32) return false;
   const int nb = lime::cdiv(d->n, 16);
   if (nb > lime::resident_nb_max(d->n_dense)) return false;
   return lime::resident_lds_bytes(nb, resident_variant(d), d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
@@ -70,7 +75,8 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
   if (use_resident(d)) return 0;
-  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d));
+  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
+                                             (d->flags & LIME_F_RENORM_TRACE) != 0);
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
@@ -88,7 +94,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   if ((d->obs != nullptr || d->states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: recording needs nout >= 1 and rec_capacity >= 0");
   if (p->step_offset < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: step_offset < 0");
-  if ((d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_3XBF16 || (d->flags & LIME_F_PREC_MASK) == 0x30)
+  if ((d->flags & LIME_F_PREC_MASK) == 0x30)
     return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: precision flag not built");
   const hipStream_t s = static_cast<hipStream_t>(stream);
 
@@ -120,9 +126,8 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   }
 
   // ---- tiled path: one launch per RK4 stage, workspace supplied by the caller
-  if (d->flags & LIME_F_RENORM_TRACE)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: trace renormalisation is built for n <= 96 only");
-  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d));
+  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
+                                                 (d->flags & LIME_F_RENORM_TRACE) != 0);
   if (d->work == nullptr || d->work_bytes < (int64_t)need)
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n = " + std::to_string(d->n) + " needs a workspace of " +
                                   std::to_string(need) + " bytes (lime_lindblad_workspace_bytes)");
@@ -145,8 +150,9 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   a.obs = static_cast<double2*>(d->obs);
   a.states = static_cast<double2*>(d->states);
   const int nout = (d->obs || d->states) ? p->nout : 0;
-  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s, tiled_tile(d),
-                                 is_fp64(d));
+  const lime::TiledPrec prec = is_fp64(d) ? lime::TiledPrec::FP64
+                              : is_split_bf16(d) ? lime::TiledPrec::SPLIT_BF16 : lime::TiledPrec::FP32;
+  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s, tiled_tile(d), prec);
   if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (tiled) launch");
   return LIME_OK;
 }
@@ -163,7 +169,7 @@ int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const voi
   if (z && c_z == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: z needs c_z");
   if (acc && c_acc == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_apply: acc needs c_acc");
   if ((d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_3XBF16 || (d->flags & LIME_F_PREC_MASK) == 0x30)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_apply: precision flag not built");
+    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_apply: built for the FP32 and FP64 variants only");
   const size_t need = (size_t)d->batch * d->n_dense * d->n * d->n * (is_fp64(d) ? sizeof(double2) : sizeof(float2));
   if (d->n_dense > 0 && (d->work == nullptr || d->work_bytes < (int64_t)need))
     return fail(LIME_ERR_ARG, "lime_lindblad_apply: dense operators need a workspace of " + std::to_string(need) + " bytes");

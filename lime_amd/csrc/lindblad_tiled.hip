@@ -12,10 +12,14 @@
 //     stage operand (double buffered so no tile reads a half-written operand), and at the
 //     last stage rho += increment in FP64, records (observables by FP64 atomics, states)
 //     and the next S = rho.
-// Two precisions share the code (their MFMA accumulator row layouts differ, see CT::row):
-// R = float (complex64 operands, v_mfma_f32_16x16x4_f32,
-// FP32 increment) and R = double (complex128 operands, v_mfma_f64_16x16x4_f64, FP64
-// everywhere -- the FP64 GPU reference).
+// Three arithmetic variants share the code (their MFMA accumulator row layouts differ, see
+// CT::row):
+//   R = float,  P = 0: complex64 operands on v_mfma_f32_16x16x4_f32, FP32 increment;
+//   R = float,  P = 1: complex64 operands split into BF16 pairs x = hi + lo as they are
+//                      staged into LDS; every real product uses the three terms
+//                      hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 (FP32 accumulate);
+//   R = double, P = 0: complex128 operands on v_mfma_f64_16x16x4_f64, FP64 everywhere
+//                      (the FP64 GPU reference).
 // The dense products T_k = L_k S of a stage come from lindblad_left_tiled, launched first.
 #include "common.hpp"
 #include "lindblad_tiled_decl.hpp"
@@ -167,6 +171,133 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
   }
 }
 
+// ---- 3xBF16 split products
+// A panel row (and B panel column) of KP complex elements is stored as 96 BF16 "slots"
+// along the MFMA reduction axis: slots 4k..4k+3 carry the main terms of element k, slots
+// 64+2k, 64+2k+1 the hi*lo terms.  With a = ar + i ai, b = br + i bi and x = x_hi + x_lo:
+//   A  slots : (ar_hi, ar_lo, ai_hi, ai_lo)       | (ar_hi, ai_hi)
+//   Br slots : (br_hi, br_hi, -bi_hi, -bi_hi)     | (br_lo, -bi_lo)     -> sum = Re(a b) - lo*lo
+//   Bi slots : (bi_hi, bi_hi,  br_hi,  br_hi)     | (bi_lo,  br_lo)     -> sum = Im(a b) - lo*lo
+// so one 16x16 block needs 3 MFMAs (32 slots each) for the real and 3 for the imaginary
+// part per panel.  A lane of group g feeds slots 32m + 8g .. +7 of MFMA m for both operands,
+// which pairs A and B slot by slot whatever order the instruction sums them in.
+constexpr int BP = 104;         // LDS pitch (BF16 slots) of a split row/column: 96 + 8, 16-byte aligned
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned bf16_rne(float x) {        // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ void split_bf16(float x, unsigned& hi, unsigned& lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - __uint_as_float(hi << 16));
+}
+
+__device__ __forceinline__ void put_a(unsigned short* row, int k, float2 v) {
+  unsigned rh, rl, ih, il;
+  split_bf16(v.x, rh, rl);
+  split_bf16(v.y, ih, il);
+  *reinterpret_cast<uint2*>(row + 4 * k) = make_uint2(rh | (rl << 16), ih | (il << 16));
+  *reinterpret_cast<unsigned*>(row + 64 + 2 * k) = rh | (ih << 16);
+}
+
+__device__ __forceinline__ void put_b(unsigned short* colr, unsigned short* coli, int k, float2 v) {
+  unsigned rh, rl, ih, il;
+  split_bf16(v.x, rh, rl);
+  split_bf16(v.y, ih, il);
+  const unsigned nih = ih ^ 0x8000u, nil = il ^ 0x8000u;
+  *reinterpret_cast<uint2*>(colr + 4 * k) = make_uint2(rh | (rh << 16), nih | (nih << 16));
+  *reinterpret_cast<unsigned*>(colr + 64 + 2 * k) = rl | (nil << 16);
+  *reinterpret_cast<uint2*>(coli + 4 * k) = make_uint2(ih | (ih << 16), rh | (rh << 16));
+  *reinterpret_cast<unsigned*>(coli + 64 + 2 * k) = il | (rl << 16);
+}
+
+// B element of a product term from the source operand: PLAIN b = M[k][j];
+// NEG_AH b = -conj(M[j][k]); I_AH b = i conj(M[j][k]).
+template <int MODE>
+__device__ __forceinline__ float2 b_elem(float2 m) {
+  if (MODE == PLAIN) return m;
+  if (MODE == NEG_AH) return make_float2(-m.x, m.y);
+  return make_float2(m.y, m.x);
+}
+
+template <int TILE, int MODE>
+__device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, const float2* __restrict__ Bsrc, int n,
+                                              int row0, int col0, unsigned char* smem,
+                                              floatx4 (&cr)[TILE / 32][TILE / 32],
+                                              floatx4 (&ci)[TILE / 32][TILE / 32], int tid, int wr, int wc,
+                                              int lr, int lg) {
+  constexpr int BW = TILE / 32;
+  constexpr int NR = TILE * KP / 256;
+  unsigned short* sA = reinterpret_cast<unsigned short*>(smem);
+  unsigned short* sBr = sA + TILE * BP;
+  unsigned short* sBi = sBr + TILE * BP;
+  float2 ra[NR], rb[NR];
+  load_rows<TILE>(A, n, row0, 0, ra, tid);
+  if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, 0, col0, rb, tid);
+  else load_rows<TILE>(Bsrc, n, col0, 0, rb, tid);
+  for (int k0 = 0; k0 < n; k0 += KP) {
+    __syncthreads();                          // previous panel fully consumed
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int e = tid + q * 256;
+      put_a(sA + (e / KP) * BP, e % KP, ra[q]);
+      if (MODE == PLAIN) put_b(sBr + (e % TILE) * BP, sBi + (e % TILE) * BP, e / TILE, rb[q]);
+      else put_b(sBr + (e / KP) * BP, sBi + (e / KP) * BP, e % KP, b_elem<MODE>(rb[q]));
+    }
+    __syncthreads();
+    if (k0 + KP < n) {                        // prefetch the next panel into registers
+      load_rows<TILE>(A, n, row0, k0 + KP, ra, tid);
+      if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, k0 + KP, col0, rb, tid);
+      else load_rows<TILE>(Bsrc, n, col0, k0 + KP, rb, tid);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      u16x8 x[BW], yr[BW], yi[BW];
+#pragma unroll
+      for (int p = 0; p < BW; ++p) {
+        x[p] = *reinterpret_cast<const u16x8*>(sA + (wr * (TILE / 2) + p * 16 + lr) * BP + 32 * m + 8 * lg);
+        const int c = (wc * (TILE / 2) + p * 16 + lr) * BP + 32 * m + 8 * lg;
+        yr[p] = *reinterpret_cast<const u16x8*>(sBr + c);
+        yi[p] = *reinterpret_cast<const u16x8*>(sBi + c);
+      }
+#pragma unroll
+      for (int p = 0; p < BW; ++p)
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+          const bf16x8 a = __builtin_bit_cast(bf16x8, x[p]);
+          cr[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, yr[q]), cr[p][q], 0, 0, 0);
+          ci[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, yi[q]), ci[p][q], 0, 0, 0);
+        }
+    }
+  }
+}
+
+// LDS bytes of a GEMM-staging area.
+template <int TILE, typename R, int P>
+constexpr int smem_bytes() {
+  return P == 1 ? 3 * TILE * BP * 2 : (TILE * KPP + smem_b<TILE>()) * (int)sizeof(typename CT<R>::C);
+}
+
+// One product term in the selected arithmetic.
+template <int TILE, int MODE, typename R, int P>
+__device__ __forceinline__ void term(const typename CT<R>::C* __restrict__ A, const typename CT<R>::C* __restrict__ B,
+                                     int n, int row0, int col0, unsigned char* smem,
+                                     typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
+                                     typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32], int tid, int wr, int wc,
+                                     int lr, int lg) {
+  if constexpr (P == 1) {
+    gemm_term_bf3<TILE, MODE>(A, B, n, row0, col0, smem, cr, ci, tid, wr, wc, lr, lg);
+  } else {
+    using C = typename CT<R>::C;
+    C* sA = reinterpret_cast<C*>(smem);
+    gemm_term<TILE, MODE, R>(A, B, n, row0, col0, sA, sA + TILE * KPP, cr, ci, tid, wr, wc, lr, lg);
+  }
+}
+
 __device__ __forceinline__ int tiles_per_dim(int n, int tile) { return (n + tile - 1) / tile; }
 
 // Common per-workgroup geometry.
@@ -191,9 +322,8 @@ __device__ __forceinline__ Geo geometry(int n, int tile_index) {
 }
 
 // C = H S - S H^dag + i sum_k T_k R_k^dag for operator b (set = b / group).
-template <int TILE, typename R>
-__device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const Geo& g,
-                                                typename CT<R>::C* sA, typename CT<R>::C* sB,
+template <int TILE, typename R, int P>
+__device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const Geo& g, unsigned char* smem,
                                                 typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
                                                 typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32]) {
   using C = typename CT<R>::C;
@@ -201,24 +331,23 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
   const size_t nn = (size_t)n * n;
   const C* H = static_cast<const C*>(a.heff) + (size_t)set * nn;
   const C* S = static_cast<const C*>(a.s_cur) + (size_t)b * nn;
-  gemm_term<TILE, PLAIN, R>(H, S, n, g.row0, g.col0, sA, sB, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
-  gemm_term<TILE, NEG_AH, R>(S, H, n, g.row0, g.col0, sA, sB, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  term<TILE, PLAIN, R, P>(H, S, n, g.row0, g.col0, smem, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  term<TILE, NEG_AH, R, P>(S, H, n, g.row0, g.col0, smem, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
   for (int k = 0; k < a.n_dense; ++k) {
     const C* T = static_cast<const C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
     const C* Rk = static_cast<const C*>(a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + k) * nn;
-    gemm_term<TILE, I_AH, R>(T, Rk, n, g.row0, g.col0, sA, sB, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+    term<TILE, I_AH, R, P>(T, Rk, n, g.row0, g.col0, smem, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
   }
 }
 
 }  // namespace
 
 // T[b][k] = L[set][k] @ S[b]  for every dense operator k.
-template <int TILE, typename R>
+template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
-  __shared__ C sA[TILE * KPP];
-  __shared__ C sB[smem_b<TILE>()];
+  __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   const int per = tiles_per_dim(a.n, TILE) * tiles_per_dim(a.n, TILE);
   const int bk = blockIdx.x / per;            // = b * n_dense + k
   const Geo g = geometry<TILE>(a.n, blockIdx.x % per);
@@ -231,9 +360,9 @@ __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
-  gemm_term<TILE, PLAIN, R>(static_cast<const C*>(a.lops) + ((size_t)set * a.n_dense + k) * nn,
-                            static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, g.row0, g.col0, sA, sB,
-                            cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  term<TILE, PLAIN, R, P>(static_cast<const C*>(a.lops) + ((size_t)set * a.n_dense + k) * nn,
+                          static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, g.row0, g.col0, smem, cr, ci,
+                          g.tid, g.wr, g.wc, g.lr, g.lg);
   C* T = static_cast<C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
 #pragma unroll
   for (int p = 0; p < BW; ++p)
@@ -248,12 +377,11 @@ __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
 }
 
 // One RK4 stage for every operator of the batch.
-template <int TILE, typename R>
+template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
-  __shared__ C sA[TILE * KPP];
-  __shared__ C sB[smem_b<TILE>()];
+  __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   __shared__ double2 sRed[4];
   const int n = a.n;
   const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
@@ -268,7 +396,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
-  generator_gemms<TILE, R>(a, b, g, sA, sB, cr, ci);
+  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci);
 
   // ---- RK4 stage epilogue
   const int st = a.stage;
@@ -318,7 +446,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
           rho[o] = r;
           rfin[p][q][v] = r;
           Sn[o] = CT<R>::make(r.x, r.y);
-          if (rec && a.states) a.states[((size_t)b * a.rec_capacity + a.rec) * nn + o] = r;
+          if (rec && a.states && !(a.flags & 1)) a.states[((size_t)b * a.rec_capacity + a.rec) * nn + o] = r;
         }
       }
   if (rec && a.obs) {
@@ -359,12 +487,11 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
 // One application of the generator to every operator of the batch, with a flexible
 // epilogue: V = f(s_cur); y = c_lx V + c_z z; y_op = y in the operand precision;
 // acc += c_acc V.
-template <int TILE, typename R>
+template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
-  __shared__ C sA[TILE * KPP];
-  __shared__ C sB[smem_b<TILE>()];
+  __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   const int n = a.n;
   const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
   const int b = blockIdx.x / per;
@@ -377,7 +504,7 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
-  generator_gemms<TILE, R>(a, b, g, sA, sB, cr, ci);
+  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci);
   const C* Dm = a.dmat ? static_cast<const C*>(a.dmat) + (size_t)set * nn : nullptr;
   const C* X = static_cast<const C*>(a.s_cur);
   C* yop = static_cast<C*>(a.y64);
@@ -475,8 +602,8 @@ static hipError_t apply_impl(TiledArgs a, hipStream_t stream) {
   const int tpd = (a.n + TILE - 1) / TILE;
   const int per = tpd * tpd;
   if (a.n_dense > 0)
-    hipLaunchKernelGGL((lindblad_left_tiled<TILE, R>), dim3(per * a.batch * a.n_dense), dim3(256), 0, stream, a);
-  hipLaunchKernelGGL((lindblad_apply_tiled<TILE, R>), dim3(per * a.batch), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((lindblad_left_tiled<TILE, R, 0>), dim3(per * a.batch * a.n_dense), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((lindblad_apply_tiled<TILE, R, 0>), dim3(per * a.batch), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -485,7 +612,7 @@ hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64) {
   return tile == 64 ? apply_impl<64, float>(a, stream) : apply_impl<32, float>(a, stream);
 }
 
-template <int TILE, typename R>
+template <int TILE, typename R, int P>
 static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) {
   const int tpd = (a.n + TILE - 1) / TILE;
   const int per = tpd * tpd;
@@ -494,12 +621,55 @@ static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) 
     a.s_cur = (st % 2 == 0) ? s0 : s1;
     a.s_next = (st % 2 == 0) ? s1 : s0;
     if (a.n_dense > 0) {
-      hipLaunchKernelGGL((lindblad_left_tiled<TILE, R>), dim3(per * a.batch * a.n_dense), dim3(256), 0,
+      hipLaunchKernelGGL((lindblad_left_tiled<TILE, R, P>), dim3(per * a.batch * a.n_dense), dim3(256), 0,
                          stream, a);
     }
-    hipLaunchKernelGGL((lindblad_stage_tiled<TILE, R>), dim3(per * a.batch), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((lindblad_stage_tiled<TILE, R, P>), dim3(per * a.batch), dim3(256), 0, stream, a);
   }
   return hipGetLastError();
+}
+
+// Trace renormalisation after a step (flag bit 0; density matrices only): the stage-3
+// kernels have written rho and accumulated the step's observables.  renorm_trace computes
+// s_b = 1 / Re Tr rho_b in FP64 (one workgroup per operator) and rescales the observables
+// recorded at this step; renorm_scale applies s_b to rho, the next stage operand and the
+// recorded state.  Observables and states are therefore recorded after renormalisation.
+__global__ void __launch_bounds__(256) renorm_trace(TiledArgs a) {
+  __shared__ double part[4];
+  const int b = blockIdx.x, n = a.n;
+  const double2* rho = a.rho + (size_t)b * n * n;
+  double t = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) t += rho[(size_t)i * n + i].x;
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  const double s = 1.0 / (part[0] + part[1] + part[2] + part[3]);
+  if (threadIdx.x == 0) a.tscale[b] = s;
+  if (a.rec >= 0 && a.obs)
+    for (int e = threadIdx.x; e < a.n_eops; e += 256) {
+      double2* o = a.obs + ((size_t)b * a.rec_capacity + a.rec) * a.n_eops + e;
+      o->x *= s;
+      o->y *= s;
+    }
+}
+
+template <typename C>
+__global__ void renorm_scale(TiledArgs a, C* __restrict__ s0) {
+  const size_t nn = (size_t)a.n * a.n;
+  const size_t total = (size_t)a.batch * nn;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = i / nn;
+    const double s = a.tscale[b];
+    double2 r = a.rho[i];
+    r.x *= s;
+    r.y *= s;
+    a.rho[i] = r;
+    C v;
+    v.x = r.x;
+    v.y = r.y;
+    s0[i] = v;
+    if (a.rec >= 0 && a.states) a.states[(b * a.rec_capacity + a.rec) * nn + (i - b * nn)] = r;
+  }
 }
 
 template <typename C>
@@ -513,10 +683,11 @@ __global__ void init_stage_operand(const double2* __restrict__ rho, C* __restric
   }
 }
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64) {
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm) {
   const size_t nn = (size_t)n * n;
   const size_t elem = fp64 ? sizeof(double2) : sizeof(float2);
-  return (size_t)batch * nn * elem * (3 + (size_t)n_dense);   // S0, S1, inc, T_k
+  const size_t ops = (size_t)batch * nn * elem * (3 + (size_t)n_dense);   // S0, S1, inc, T_k
+  return renorm ? ops + (size_t)batch * sizeof(double) : ops;             // + trace scales
 }
 
 int tiled_pick(int batch, int n) {
@@ -524,7 +695,7 @@ int tiled_pick(int batch, int n) {
   return ((size_t)batch * t64 * t64 >= 512) ? 64 : 32;
 }
 
-template <typename R>
+template <typename R, int P>
 static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
                           int tile) {
   using C = typename CT<R>::C;
@@ -535,6 +706,8 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
   a.inc = inc;
   a.tbuf = inc + (size_t)a.batch * nn;
   const size_t cnt = (size_t)a.batch * nn;
+  // trace scales follow the operand buffers (S0, S1, inc, T_k)
+  a.tscale = reinterpret_cast<double*>(static_cast<C*>(a.tbuf) + (size_t)a.batch * a.n_dense * nn);
   hipLaunchKernelGGL(init_stage_operand<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, a.rho,
                      s0, cnt);
   for (int s = 0; s < nsteps; ++s) {
@@ -543,16 +716,25 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
     if (nout > 0 && (a.obs || a.states) && gstep % nout == 0 && gstep / nout - 1 < a.rec_capacity)
       a.rec = gstep / nout - 1;
     // stage 3 writes S = rho into s0 (stages alternate s0 -> s1 -> s0 -> s1 -> s0)
-    hipError_t e = (tile == 64) ? run_step<64, R>(a, stream, s0, s1) : run_step<32, R>(a, stream, s0, s1);
+    hipError_t e = (tile == 64) ? run_step<64, R, P>(a, stream, s0, s1) : run_step<32, R, P>(a, stream, s0, s1);
     if (e != hipSuccess) return e;
+    if (a.flags & 1) {
+      hipLaunchKernelGGL(renorm_trace, dim3(a.batch), dim3(256), 0, stream, a);
+      const size_t blocks = (cnt + 255) / 256;
+      hipLaunchKernelGGL(renorm_scale<C>, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0, stream,
+                         a, s0);
+    }
   }
   return hipGetLastError();
 }
 
 hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
-                     int tile, bool fp64) {
-  return fp64 ? run_all<double>(a, nsteps, step_offset, nout, work, stream, tile)
-              : run_all<float>(a, nsteps, step_offset, nout, work, stream, tile);
+                     int tile, TiledPrec prec) {
+  switch (prec) {
+    case TiledPrec::FP64: return run_all<double, 0>(a, nsteps, step_offset, nout, work, stream, tile);
+    case TiledPrec::SPLIT_BF16: return run_all<float, 1>(a, nsteps, step_offset, nout, work, stream, tile);
+    default: return run_all<float, 0>(a, nsteps, step_offset, nout, work, stream, tile);
+  }
 }
 
 }  // namespace lime

@@ -24,6 +24,7 @@ struct TiledArgs {
   void* s_next;
   void* inc;
   void* tbuf;
+  double* tscale;          // per-operator 1 / Re Tr rho of the trace renormalisation
   // generator application (lindblad_apply_tiled): V = L(s_cur);
   //   y = c_lx V + c_z z (complex128), y64 = copy of y in the operand precision,
   //   acc += c_acc V
@@ -34,10 +35,14 @@ struct TiledArgs {
   double2 c_lx, c_z, c_acc;
 };
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64);
+// GEMM arithmetic of the tiled kernels: complex64 on FP32 MFMA, complex64 as 3-term BF16
+// split products on BF16 MFMA, complex128 on FP64 MFMA.
+enum class TiledPrec { FP32 = 0, SPLIT_BF16 = 1, FP64 = 2 };
+
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm);
 int tiled_pick(int batch, int n);
 hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
-                     int tile, bool fp64);
+                     int tile, TiledPrec prec);
 hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64);
 
 struct ContractArgs {

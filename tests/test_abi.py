@@ -67,11 +67,17 @@ def test_argument_validation_without_gpu(lib):
     assert _call(lib, obs=0x3000) == _lib.LIME_ERR_ARG         # obs without e_ops
     assert _call(lib, flags=_lib.LIME_F_PREC_FP64) == _lib.LIME_ERR_ARG   # tiled: needs workspace
     assert _call(lib, flags=0x30) == _lib.LIME_ERR_UNSUPPORTED
+    # 3xBF16 runs on the tiled kernels at every n: it needs a workspace too
+    assert _call(lib, flags=_lib.LIME_F_PREC_3XBF16) == _lib.LIME_ERR_ARG
+    assert b"workspace" in lib.lime_last_error()
     # n > 96 runs the tiled kernels, which need a caller-provided workspace
     assert _call(lib, n=97) == _lib.LIME_ERR_ARG
     assert b"workspace" in lib.lime_last_error()
-    assert _call(lib, n=200, work=0x4000, work_bytes=1 << 40,
-                 flags=_lib.LIME_F_RENORM_TRACE) == _lib.LIME_ERR_UNSUPPORTED
+    # trace renormalisation on the tiled kernels needs 8 bytes per operator more workspace
+    plain = 1 * 200 * 200 * 8 * 3
+    assert _call(lib, n=200, work=0x4000, work_bytes=plain,
+                 flags=_lib.LIME_F_RENORM_TRACE) == _lib.LIME_ERR_ARG
+    assert b"workspace" in lib.lime_last_error()
     with pytest.raises(_lib.LimeError):
         _lib.check(_call(lib, n=200))
 
@@ -84,3 +90,7 @@ def test_workspace_query(lib):
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 64 * 64 * 8 * 3
     d.flags, d.n, d.n_dense = 0, 130, 2
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 130 * 130 * 8 * 5
+    d.flags, d.n, d.n_dense = _lib.LIME_F_PREC_3XBF16, 16, 0                 # complex64 workspace
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 16 * 16 * 8 * 3
+    d.flags = _lib.LIME_F_PREC_3XBF16 | _lib.LIME_F_RENORM_TRACE             # + trace scales
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 16 * 16 * 8 * 3 + 4 * 8

@@ -256,3 +256,62 @@ def test_every_launch_variant_matches_oracle(n, kd):
             print(f"variant n={n} kd={kd} v={v} b={b} rel err {err:.3e}")
             assert err < 2e-7
             assert np.max(np.abs(obs[b].cpu().numpy() - ref_obs)) < 1e-6 * max(1.0, np.max(np.abs(ref_obs)))
+
+
+@pytest.mark.parametrize("n,kd", [(20, 0), (65, 0), (70, 2), (130, 1)])
+def test_split_bf16_variant_matches_oracle(n, kd):
+    """3xBF16 split products: each operand x = hi + lo (BF16 pair, round to nearest even),
+    products hi*hi + hi*lo + lo*hi.  Representation error <= 2^-17 |x| per operand and the
+    dropped lo*lo term <= 2^-18 |a b|, so one RHS is accurate to ~1e-5 relative; over 10
+    steps of dt = 0.02 with ||H|| = 1 the state error stays below 1e-5 * max|rho|.  Keeping
+    only hi*hi (a plausible slip) would give ~1e-3."""
+    rng = np.random.default_rng(700 + n)
+    H = rand_herm(rng, n)
+    c_ops = [0.1 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(kd)]
+    c_ops += [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_dm(rng, n), rand_c(rng, n, n) / n])
+    A = [rand_herm(rng, n)]
+    dt, nt = 0.02, 10
+    refs = [ol.propagate(H, X0[b], c_ops, dt, nt, A, nout=5) for b in range(2)]
+    errs = {}
+    for prec in ("3xbf16", "fp32"):
+        prop = _prop()(H, c_ops, group=2, precision=prec, force_tiled=True)
+        rho, obs, _ = prop.run(X0, dt, nt, e_ops=A, nout=5)
+        errs[prec] = []
+        for b in range(2):
+            ref, ref_obs, _ = refs[b]
+            errs[prec].append(np.max(np.abs(rho[b].cpu().numpy() - ref)) / np.max(np.abs(ref)))
+            if prec == "3xbf16":
+                assert np.max(np.abs(obs[b].cpu().numpy() - ref_obs)) < 1e-5 * np.max(np.abs(ref_obs))
+    print(f"n={n} kd={kd} rel err 3xbf16 {max(errs['3xbf16']):.3e} fp32 {max(errs['fp32']):.3e}")
+    assert max(errs["3xbf16"]) < 1e-5
+    assert max(errs["fp32"]) < 2e-7
+
+
+@pytest.mark.parametrize("n,prec", [(20, "fp32"), (130, "fp32"), (130, "fp64"), (100, "3xbf16")])
+def test_tiled_trace_renormalisation_records_after_scaling(n, prec):
+    """Trace renormalisation on the per-stage kernels: Re Tr rho = 1 after every step, the
+    recorded states and observables are those of the renormalised state, and the physics
+    agrees with the (trace-conserving) FP64 oracle."""
+    rng = np.random.default_rng(40 + n)
+    H = rand_herm(rng, n)
+    c_ops = [0.2 * rand_c(rng, n, n) / np.sqrt(n)] + [np.sqrt(0.05) * np.diag(np.eye(n)[i]) for i in range(n)]
+    rho0 = rand_dm(rng, n)
+    E = [np.diag(np.eye(n)[i]).astype(complex) for i in (0, n // 2)]
+    dt, nt = 0.05, 40
+    prop = _prop()(H, c_ops, precision=prec, force_tiled=True)
+    rho, obs, states = prop.run(rho0, dt, nt, e_ops=E, nout=8, store_states=True, renorm=True)
+    r = rho[0].cpu().numpy()
+    st = states[0].cpu().numpy()
+    ob = obs[0].cpu().numpy()
+    assert abs(np.trace(r).real - 1) < 1e-13
+    for k in range(st.shape[0]):
+        assert abs(np.trace(st[k]).real - 1) < 1e-13
+        for e in range(len(E)):
+            assert abs(ob[k, e] - np.trace(E[e] @ st[k])) < 1e-12
+    assert np.allclose(st[-1], r, rtol=0, atol=0)
+    ref, _, _ = ol.propagate(H, rho0, c_ops, dt, nt)
+    tol = {"fp32": 2e-7, "fp64": 1e-13, "3xbf16": 2e-6}[prec]
+    err = np.max(np.abs(r - ref)) / np.max(np.abs(ref))
+    print(f"tiled renorm n={n} {prec}: rel err {err:.3e}")
+    assert err < tol
