@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--instances", type=int, default=2048, help="ragged: number of instances")
     ap.add_argument("--check-steps", type=int, default=100,
                     help="dense512: RK4 steps of the FP64 GPU reference comparison")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a copy-like kernel (y = -x, float32) over MIB MiB three "
+                         "times: the reference reading for the HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -104,6 +107,17 @@ def main():
 
     import torch
     torch.cuda.set_device(torch.device("cuda", 0))
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"kernel": "torch.neg float32 (read once, write once)",
+                                               "read_bytes": nbytes, "write_bytes": nbytes,
+                                               "launches": 3}}), flush=True)
+        return
     if args.workload != "lattice65":
         rk_default = {"ragged": 10, "dense512": 10, "gradient": 0}[args.workload]
         if args.rk_per_step == ap.get_default("rk_per_step"):

@@ -27,8 +27,7 @@ bool is_fp64(const lime_lindblad_batch* d) { return (d->flags & LIME_F_PREC_MASK
 
 // Resident launch-shape variant requested by d->variant (0 = default).
 int resident_variant(const lime_lindblad_batch* d) {
-  const int nb = lime::cdiv(d->n, 16);
-  return (d->variant > 0 && d->variant < lime::resident_variants(nb)) ? d->variant : 0;
+  return (d->variant > 0 && d->variant < lime::resident_variants(d->n, d->n_dense)) ? d->variant : 0;
 }
 
 bool is_split_bf16(const lime_lindblad_batch* d) {
@@ -41,7 +40,7 @@ bool use_resident(const lime_lindblad_batch* d) {
   if ((d->flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32) return false;
   const int nb = lime::cdiv(d->n, 16);
   if (nb > lime::resident_nb_max(d->n_dense)) return false;
-  return lime::resident_lds_bytes(nb, resident_variant(d), d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
+  return lime::resident_lds_bytes(d->n, resident_variant(d), d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
 }
 
 // Tiled tile edge requested by d->variant (1 -> 32, 2 -> 64), else the heuristic.
@@ -68,7 +67,7 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1) return 0;
   lime_lindblad_batch probe = *d;
   probe.variant = 0;
-  if (use_resident(&probe)) return lime::resident_variants(lime::cdiv(d->n, 16));
+  if (use_resident(&probe)) return lime::resident_variants(d->n, d->n_dense);
   return 3;   // tiled: 0 = heuristic, 1 = 32x32 tiles, 2 = 64x64 tiles
 }
 
@@ -120,7 +119,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
     a.nout = (d->obs || d->states) ? p->nout : 0;
     a.step_offset = p->step_offset;
     a.rec_capacity = p->rec_capacity;
-    hipError_t e = lime::resident_launch(a, lime::cdiv(d->n, 16), resident_variant(d), s);
+    hipError_t e = lime::resident_launch(a, resident_variant(d), s);
     if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (resident) launch");
     return LIME_OK;
   }

@@ -41,10 +41,11 @@ __device__ __forceinline__ void stage_operator(float2* dst, const float2* __rest
 // Software-pipelined complex product over LDS operands for the wave's tiles:
 //   MODE 0: C += A B          (A rows from sA, B[k][j] = sB[k][j])
 //   MODE 2: C += i A M^dag    (A rows from sA, B[k][j] = conj(sM[j][k]), sM = sB)
-// sA / sB may be read up to 4 columns / rows past NP (spare LDS after each buffer).
+// The reduction runs over k < ke (operands are zero beyond n).  sA / sB may be read up to
+// 4 columns / rows past ke (padding, or the spare LDS after each buffer).
 template <int NP, int P, int BPW, int MODE>
 __device__ __forceinline__ void lds_gemm(const float2* sA, const float2* sB, const int (&trow)[BPW],
-                                         const int (&tcol)[BPW], int lr, int lg, floatx4 (&cr)[BPW],
+                                         const int (&tcol)[BPW], int lr, int lg, int ke, floatx4 (&cr)[BPW],
                                          floatx4 (&ci)[BPW]) {
   int oA[BPW], oB[BPW];
 #pragma unroll
@@ -59,7 +60,7 @@ __device__ __forceinline__ void lds_gemm(const float2* sA, const float2* sB, con
     y[u] = sB[oB[u]];
   }
 #pragma unroll 2
-  for (int k0 = 0; k0 < NP; k0 += 4) {
+  for (int k0 = 0; k0 < ke; k0 += 4) {
     float2 xn[BPW], yn[BPW];
 #pragma unroll
     for (int u = 0; u < BPW; ++u) {
@@ -91,14 +92,29 @@ __device__ __forceinline__ void lds_gemm(const float2* sA, const float2* sB, con
 
 }  // namespace
 
-template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC>
+// Bordered sizes (n = 16 m + 1, no dense jump operators): the last index is the border.
+// The core block (rows/columns 0..16m-1) runs on MFMA over the reduction range 0..16m+3,
+// which already contains the border terms H[i][16m] S[16m][j] of core outputs; the 2*16m+1
+// border outputs (row 16m and column 16m) are 65-term-or-shorter complex dot products on
+// VALU, each split over BG lanes.  The padded size of the plain kernel would spend
+// (m+1)^2 tiles on these outputs instead of m^2.
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(MINW)))
 lindblad_rk4_resident(ResidentArgs a) {
   constexpr int NP = NB * 16;
   constexpr int P = NP + 1;
   constexpr int T = W * 64;
-  constexpr int NT = NB * NB;
+  constexpr int TG = BRD ? NB - 1 : NB;     // MFMA tile grid edge
+  constexpr int NT = TG * TG;
   static_assert(W * BPW >= NT, "every tile needs an owner");
+  constexpr int NC = 16 * (NB - 1);          // bordered: core size = border index
+  constexpr int NBP = BRD ? 2 * NC + 1 : 0;  // bordered: number of border outputs
+  constexpr int BG = (T >= 4 * NBP) ? 4 : (T >= 2 * NBP) ? 2 : 1;
+  static_assert(!BRD || T >= NBP, "every border output needs an owner");
+  static_assert(!(BRD && DENSE), "bordered kernels are built without dense jump operators");
+  // per-lane caches of D and of a single observable (needs the register budget of one
+  // tile per lane at <= 4 waves per SIMD, or of at most two waves per SIMD)
+  constexpr bool CACHE = (BPW == 1 && MINW <= 4) || MINW <= 2;
 
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   float2* sH = reinterpret_cast<float2*>(lds_raw);
@@ -119,8 +135,11 @@ lindblad_rk4_resident(ResidentArgs a) {
   const int nn = n * n;   // n <= 96: element offsets fit in 32 bits
   const int lr = lane & 15;
   const int lg = lane >> 4;
+  // reduction extent: operands are zero beyond n, so round n up to the MFMA k-step
+  const int ke = BRD ? NC + 4 : ((n + 3) & ~3);
 
-  // ---- stage H_eff into LDS (zero padded to NP)
+  // ---- stage H_eff into LDS (zero padded to NP); bordered: S is zeroed too, since its
+  // padding is not covered by any owner
   {
     const float2* H = a.heff + (size_t)set * nn;
     for (int idx = tid; idx < NP * NP; idx += T) {
@@ -128,8 +147,10 @@ lindblad_rk4_resident(ResidentArgs a) {
       float2 v = make_float2(0.f, 0.f);
       if (i < n && j < n) v = H[(size_t)i * n + j];
       sH[i * P + j] = v;
+      if (BRD) sS[i * P + j] = make_float2(0.f, 0.f);
     }
   }
+  if (BRD) __syncthreads();
 
   // ---- tile ownership: first row / column of each owned tile (wave-uniform)
   int trow[BPW], tcol[BPW];
@@ -139,12 +160,32 @@ lindblad_rk4_resident(ResidentArgs a) {
     const int t = w + u * W;
     tval[u] = t < NT;
     const int tt = tval[u] ? t : NT - 1;   // idle slots shadow a real tile, never store
-    trow[u] = (tt / NB) * 16;
-    tcol[u] = (tt % NB) * 16;
+    trow[u] = (tt / TG) * 16;
+    tcol[u] = (tt % TG) * 16;
   }
+  // ---- bordered: border output (bi, bj) of lane group tid / BG; lane bq of the group
+  // sums k = bq, bq + BG, ...; every lane of the group ends with the same FP64 state
+  int bi = 0, bj = 0, bq = 0;
+  bool bown = false;
+  if (BRD) {
+    const int grp = tid / BG;
+    bq = tid % BG;
+    bown = grp < NBP;
+    const int gg = bown ? grp : 0;
+    if (gg <= NC) {
+      bi = NC;
+      bj = gg;
+    } else {
+      bi = gg - NC - 1;
+      bj = NC;
+    }
+  }
+  const bool bwrite = bown && bq == 0;
 
   double2 rho[BPW][4];   // FP64 state
   float2 inc[BPW][4];    // RK4 increment dt*(k1 + 2k2 + 2k3 + k4)/6, summed in FP32
+  double2 rho_b = make_double2(0.0, 0.0);   // bordered: state of the border output
+  float2 inc_b = make_float2(0.f, 0.f);
   {
     const double2* R = a.rho + (size_t)b * nn;
 #pragma unroll
@@ -160,12 +201,44 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (tval[u]) sS[i * P + j] = make_float2((float)x.x, (float)x.y);
       }
     }
+    if (bown) {
+      rho_b = R[bi * n + bj];
+      if (bwrite) sS[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
+    }
+  }
+  // ---- register caches: D at the owned elements, the single observable's E^T
+  const bool has_d = a.dmat != nullptr;
+  float2 dc[CACHE ? BPW : 1][4];
+  float2 dc_b = make_float2(0.f, 0.f);
+  const bool ecached = CACHE && REC && a.obs != nullptr && a.n_eops == 1;
+  double2 ec[CACHE && REC ? BPW : 1][4];
+  double2 ec_b = make_double2(0.0, 0.0);
+  if constexpr (CACHE) {
+    const float2* Dm = a.dmat + (size_t)set * nn;
+    const double2* E = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
+#pragma unroll
+    for (int u = 0; u < BPW; ++u) {
+      const int j = tcol[u] + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = trow[u] + 4 * lg + v;
+        const bool in = tval[u] && i < n && j < n;
+        dc[u][v] = (has_d && in) ? Dm[i * n + j] : make_float2(0.f, 0.f);
+        if constexpr (REC) ec[u][v] = (ecached && in) ? E[i * n + j] : make_double2(0.0, 0.0);
+      }
+    }
+    if (bown) {
+      if (has_d) dc_b = Dm[bi * n + bj];
+      if (ecached) ec_b = E[bi * n + bj];
+    }
+  } else if (BRD && bown) {
+    if (has_d) dc_b = a.dmat[(size_t)set * nn + bi * n + bj];
+    if (REC && ecached) ec_b = a.eops_t[(size_t)(b / a.eop_group) * a.n_eops * nn + bi * n + bj];
   }
   __syncthreads();
 
   const double dt = a.dt;
   const bool renorm = (a.flags & 1) != 0;
-  const bool has_d = a.dmat != nullptr;
   const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
 
   for (int s = 0; s < a.nsteps; ++s) {
@@ -203,9 +276,9 @@ lindblad_rk4_resident(ResidentArgs a) {
           h[u] = sH[oHc[u]];
         }
 #pragma unroll 2
-        for (int k0 = 0; k0 < NP; k0 += 4) {
+        for (int k0 = 0; k0 < ke; k0 += 4) {
           float2 xn[BPW], yn[BPW], zn[BPW], hn[BPW];
-          // prefetch step k0+4; at k0+4 == NP this reads the padding and is discarded
+          // prefetch step k0+4; at k0+4 == ke this reads padding or spare rows, discarded
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
             xn[u] = sH[oR[u] + k0 + 4];
@@ -233,6 +306,23 @@ lindblad_rk4_resident(ResidentArgs a) {
           }
         }
       }
+      // bordered: C at the border output, (H S - S H^dag)_ij over k = 0..NC
+      float br = 0.f, bm = 0.f;
+      if (BRD) {
+        if (bown) {
+          for (int k = bq; k <= NC; k += BG) {
+            const float2 hh = sH[bi * P + k], ss = sS[k * P + bj];
+            const float2 zz = sS[bi * P + k], gg = sH[bj * P + k];
+            br += hh.x * ss.x - hh.y * ss.y - (zz.x * gg.x + zz.y * gg.y);
+            bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
+          }
+        }
+#pragma unroll
+        for (int off = 1; off < BG; off <<= 1) {
+          br += __shfl_xor(br, off, 64);
+          bm += __shfl_xor(bm, off, 64);
+        }
+      }
       // C += i sum_k (L_k S) R_k^dag, every factor staged through LDS (sL) so the fragment
       // loops are the same pipelined LDS reads as the H_eff products.
       for (int kk = 0; DENSE && kk < a.n_dense; ++kk) {
@@ -246,7 +336,7 @@ lindblad_rk4_resident(ResidentArgs a) {
           tr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
           ti[u] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
-        lds_gemm<NP, P, BPW, 0>(sL, sS, trow, tcol, lr, lg, tr, ti);    // T = L S
+        lds_gemm<NP, P, BPW, 0>(sL, sS, trow, tcol, lr, lg, ke, tr, ti);    // T = L S
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
           if (!tval[u]) continue;
@@ -259,7 +349,7 @@ lindblad_rk4_resident(ResidentArgs a) {
           stage_operator<NP, P, T>(sL, a.rops + ((size_t)set * a.n_dense + kk) * nn, n, tid);
         }
         __syncthreads();
-        lds_gemm<NP, P, BPW, 2>(sT, sL, trow, tcol, lr, lg, cr, ci);    // C += i T R^dag
+        lds_gemm<NP, P, BPW, 2>(sT, sL, trow, tcol, lr, lg, ke, cr, ci);    // C += i T R^dag
       }
       // k = -i C + D o S, formed in place: afterwards kr = ci, ki = -cr.
       if (has_d) {
@@ -271,13 +361,23 @@ lindblad_rk4_resident(ResidentArgs a) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int i = trow[u] + 4 * lg + v;
-            float2 d = make_float2(0.f, 0.f);
-            if (i < n && j < n) d = Dm[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+            float2 d;
+            if constexpr (CACHE) {
+              d = dc[u][v];
+            } else {
+              d = make_float2(0.f, 0.f);
+              if (i < n && j < n) d = Dm[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+            }
             const float2 x = sS[i * P + j];
             ci[u][v] += d.x * x.x - d.y * x.y;
             cr[u][v] -= d.x * x.y + d.y * x.x;
-            __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
+            if (!CACHE) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
           }
+        }
+        if (BRD) {
+          const float2 x = sS[bi * P + bj];
+          bm += dc_b.x * x.x - dc_b.y * x.y;
+          br -= dc_b.x * x.y + dc_b.y * x.x;
         }
       }
       __syncthreads();  // every wave has finished reading S
@@ -305,6 +405,20 @@ lindblad_rk4_resident(ResidentArgs a) {
           }
         }
       }
+      if (BRD) {
+        const float kr = bm, ki = -br;
+        float2 ic = (st == 0) ? make_float2(0.f, 0.f) : inc_b;
+        ic.x = fmaf(cacc, kr, ic.x);
+        ic.y = fmaf(cacc, ki, ic.y);
+        if (st < 3) {
+          inc_b = ic;
+          if (bwrite)
+            sS[bi * P + bj] = make_float2((float)(rho_b.x + cnext * (double)kr), (float)(rho_b.y + cnext * (double)ki));
+        } else {
+          rho_b.x += (double)ic.x;
+          rho_b.y += (double)ic.y;
+        }
+      }
       if (st < 3) {
         __syncthreads();  // S of the next stage is complete
         continue;
@@ -318,6 +432,7 @@ lindblad_rk4_resident(ResidentArgs a) {
 #pragma unroll
           for (int v = 0; v < 4; ++v)
             if (tval[u] && trow[u] == tcol[u] && 4 * lg + v == lr) tr += rho[u][v].x;
+        if (BRD && bwrite && bi == bj) tr += rho_b.x;
         tr = wave_sum(tr);
         if (lane == 0) sRed[w].x = tr;
         __syncthreads();
@@ -331,6 +446,8 @@ lindblad_rk4_resident(ResidentArgs a) {
             rho[u][v].x *= inv;
             rho[u][v].y *= inv;
           }
+        rho_b.x *= inv;
+        rho_b.y *= inv;
       }
 #pragma unroll
       for (int u = 0; u < BPW; ++u) {
@@ -340,6 +457,7 @@ lindblad_rk4_resident(ResidentArgs a) {
           sS[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] =
               make_float2((float)rho[u][v].x, (float)rho[u][v].y);
       }
+      if (BRD && bwrite) sS[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
 
       const int g = a.step_offset + s + 1;
       const bool rec_now = record_any && (g % a.nout) == 0 && (g / a.nout - 1) < a.rec_capacity;
@@ -357,12 +475,22 @@ lindblad_rk4_resident(ResidentArgs a) {
               for (int v = 0; v < 4; ++v) {
                 const int i = trow[u] + 4 * lg + v;
                 if (tval[u] && i < n && j < n) {
-                  const double2 ev = E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+                  double2 ev;
+                  if constexpr (CACHE && REC) {
+                    ev = ecached ? ec[u][v] : E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+                  } else {
+                    ev = E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+                  }
                   pr += ev.x * rho[u][v].x - ev.y * rho[u][v].y;
                   pi += ev.x * rho[u][v].y + ev.y * rho[u][v].x;
                 }
-                __builtin_amdgcn_sched_barrier(0);
+                if (!CACHE) __builtin_amdgcn_sched_barrier(0);
               }
+            }
+            if (BRD && bwrite) {
+              const double2 ev = ecached ? ec_b : E[bi * n + bj];
+              pr += ev.x * rho_b.x - ev.y * rho_b.y;
+              pi += ev.x * rho_b.y + ev.y * rho_b.x;
             }
             pr = wave_sum(pr);
             pi = wave_sum(pi);
@@ -381,6 +509,7 @@ lindblad_rk4_resident(ResidentArgs a) {
               if (tval[u] && i < n && j < n) Sg[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr] = rho[u][v];
             }
           }
+          if (BRD && bwrite) Sg[bi * n + bj] = rho_b;
         }
       }
       __syncthreads();  // S of the next step (and the record partials) are complete
@@ -411,16 +540,17 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (tval[u] && i < n && j < n) R[i * n + j] = rho[u][v];
       }
     }
+    if (BRD && bwrite) R[bi * n + bj] = rho_b;
   }
 }
 
 // ------------------------------------------------------------------------ host launch
 namespace {
 
-// Launch shapes per padded size NB: W waves, BPW tiles per wave (W * BPW >= NB^2), MINW
-// the waves-per-SIMD the register budget is compiled for (512 / MINW VGPRs).  The FP64
-// state costs 16 doubles per tile per lane, so BPW sets the budget: 1 -> 6, 2 -> 4,
-// 3 -> 3, 4 -> 2.  Variant 0 is the default; the autotuner times the others.
+// Launch shapes per padded size NB: W waves, BPW tiles per wave, MINW the waves per SIMD
+// the register budget is compiled for (512 / MINW VGPRs).  The FP64 state costs 16
+// doubles per tile per lane, so BPW sets the budget.  Variant 0 is the default; the
+// autotuner times the others.  Bordered shapes (n = 16 (NB-1) + 1) own (NB-1)^2 tiles.
 struct Shape {
   int w, bpw;
 };
@@ -434,10 +564,19 @@ constexpr Shape kShapes[7][kMaxVariants] = {
     {{13, 2}, {9, 3}, {7, 4}, {0, 0}},
     {{12, 3}, {0, 0}, {0, 0}, {0, 0}},
 };
+constexpr Shape kBorderShapes[7][kMaxVariants] = {
+    {},
+    {},
+    {{4, 1}, {2, 1}, {0, 0}, {0, 0}},
+    {{4, 1}, {8, 1}, {2, 2}, {0, 0}},
+    {{9, 1}, {5, 2}, {3, 3}, {0, 0}},
+    {{16, 1}, {8, 2}, {4, 4}, {0, 0}},
+    {{13, 2}, {9, 3}, {7, 4}, {0, 0}},
+};
 
-template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC>
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD>
 hipError_t launch_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
-  auto k = lindblad_rk4_resident<NB, W, BPW, MINW, DENSE, REC>;
+  auto k = lindblad_rk4_resident<NB, W, BPW, MINW, DENSE, REC, BRD>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -445,11 +584,11 @@ hipError_t launch_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <int NB, int W, int BPW, int MINW, bool DENSE>
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool BRD>
 hipError_t launch_rec(const ResidentArgs& a, size_t lds, hipStream_t stream) {
   const bool rec = a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
-  return rec ? launch_one<NB, W, BPW, MINW, DENSE, true>(a, lds, stream)
-             : launch_one<NB, W, BPW, MINW, DENSE, false>(a, lds, stream);
+  return rec ? launch_one<NB, W, BPW, MINW, DENSE, true, BRD>(a, lds, stream)
+             : launch_one<NB, W, BPW, MINW, DENSE, false, BRD>(a, lds, stream);
 }
 
 template <int NB, int W, int BPW>
@@ -457,37 +596,75 @@ hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream) {
   static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
   constexpr int MINW = BPW == 1 ? 6 : BPW == 2 ? 4 : BPW == 3 ? 3 : 2;
   if constexpr (NB <= 4) {
-    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true>(a, lds, stream);
+    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true, false>(a, lds, stream);
   } else {
     if (a.n_dense > 0) return hipErrorInvalidValue;
   }
-  return launch_rec<NB, W, BPW, MINW, false>(a, lds, stream);
+  return launch_rec<NB, W, BPW, MINW, false, false>(a, lds, stream);
+}
+
+// Bordered shapes: one workgroup per CU at the larger sizes, so the register budget is
+// that of ceil(W / 4) waves per SIMD (at least 2 workgroups per CU below NB = 4).
+template <int NB, int W, int BPW>
+hipError_t launch_border(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+  static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
+  constexpr int PER_SIMD = (W + 3) / 4;
+  constexpr int MINW = NB <= 3 ? 2 * PER_SIMD : PER_SIMD;
+  if (a.n_dense > 0 || a.n != 16 * (NB - 1) + 1) return hipErrorInvalidValue;
+  return launch_rec<NB, W, BPW, MINW, false, true>(a, lds, stream);
+}
+
+const Shape& shape_of(int nb, int variant, bool brd) {
+  return brd ? kBorderShapes[nb][variant] : kShapes[nb][variant];
 }
 
 }  // namespace
 
 int resident_nb_max(int n_dense) { return n_dense > 0 ? 4 : 6; }
 
-int resident_variants(int nb) {
+bool resident_bordered(int n, int n_dense) { return n_dense == 0 && n >= 17 && n % 16 == 1 && n <= 81; }
+
+int resident_variants(int n, int n_dense) {
+  const int nb = cdiv(n, 16);
   if (nb < 1 || nb > 6) return 0;
+  const bool brd = resident_bordered(n, n_dense);
   int c = 0;
-  while (c < kMaxVariants && kShapes[nb][c].w > 0) ++c;
+  while (c < kMaxVariants && shape_of(nb, c, brd).w > 0) ++c;
   return c;
 }
 
-int resident_waves(int nb, int variant) { return kShapes[nb][variant].w; }
-
-size_t resident_lds_bytes(int nb, int variant, int n_dense, int n_eops) {
+size_t resident_lds_bytes(int n, int variant, int n_dense, int n_eops) {
+  const int nb = cdiv(n, 16);
   const int np = nb * 16, p = np + 1;
   const size_t buf = (size_t)np * p + 4 * p + 4;   // one operator buffer + prefetch spare rows
   size_t bytes = (np * p + (n_dense > 0 ? 3 : 1) * buf) * sizeof(float2);
-  bytes += (size_t)(1 + n_eops) * kShapes[nb][variant].w * sizeof(double2);
+  bytes += (size_t)(1 + n_eops) * shape_of(nb, variant, resident_bordered(n, n_dense)).w * sizeof(double2);
   return bytes;
 }
 
-hipError_t resident_launch(const ResidentArgs& a, int nb, int variant, hipStream_t stream) {
-  if (variant < 0 || variant >= resident_variants(nb)) return hipErrorInvalidValue;
-  const size_t lds = resident_lds_bytes(nb, variant, a.n_dense, a.n_eops);
+hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t stream) {
+  if (variant < 0 || variant >= resident_variants(a.n, a.n_dense)) return hipErrorInvalidValue;
+  const size_t lds = resident_lds_bytes(a.n, variant, a.n_dense, a.n_eops);
+  const int nb = cdiv(a.n, 16);
+  if (resident_bordered(a.n, a.n_dense)) {
+    switch (nb * 10 + variant) {
+      case 20: return launch_border<2, 4, 1>(a, lds, stream);
+      case 21: return launch_border<2, 2, 1>(a, lds, stream);
+      case 30: return launch_border<3, 4, 1>(a, lds, stream);
+      case 31: return launch_border<3, 8, 1>(a, lds, stream);
+      case 32: return launch_border<3, 2, 2>(a, lds, stream);
+      case 40: return launch_border<4, 9, 1>(a, lds, stream);
+      case 41: return launch_border<4, 5, 2>(a, lds, stream);
+      case 42: return launch_border<4, 3, 3>(a, lds, stream);
+      case 50: return launch_border<5, 16, 1>(a, lds, stream);
+      case 51: return launch_border<5, 8, 2>(a, lds, stream);
+      case 52: return launch_border<5, 4, 4>(a, lds, stream);
+      case 60: return launch_border<6, 13, 2>(a, lds, stream);
+      case 61: return launch_border<6, 9, 3>(a, lds, stream);
+      case 62: return launch_border<6, 7, 4>(a, lds, stream);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (nb * 10 + variant) {
     case 10: return launch_shape<1, 1, 1>(a, lds, stream);
     case 20: return launch_shape<2, 2, 2>(a, lds, stream);

@@ -21,9 +21,10 @@ struct ResidentArgs {
 };
 
 int resident_nb_max(int n_dense);
-int resident_variants(int nb);
-int resident_waves(int nb, int variant);
-size_t resident_lds_bytes(int nb, int variant, int n_dense, int n_eops);
-hipError_t resident_launch(const ResidentArgs& a, int nb, int variant, hipStream_t stream);
+// n = 16 m + 1 without dense jump operators runs the bordered kernels
+bool resident_bordered(int n, int n_dense);
+int resident_variants(int n, int n_dense);
+size_t resident_lds_bytes(int n, int variant, int n_dense, int n_eops);
+hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t stream);
 
 }  // namespace lime

@@ -136,11 +136,14 @@ def test_qrt_non_hermitian_operator_batch_and_groups():
         assert np.max(np.abs(obs[b] - ref_obs)) < 1e-6 * np.max(np.abs(ref_obs))
 
 
-def test_trace_renormalisation_keeps_unit_trace():
-    rng = np.random.default_rng(9)
-    n = 20
+@pytest.mark.parametrize("n,dense", [(20, True), (33, False), (65, False)])
+def test_trace_renormalisation_keeps_unit_trace(n, dense):
+    """Resident kernels; n = 33 and 65 without dense operators are the bordered ones (the
+    corner element of the border enters the trace)."""
+    rng = np.random.default_rng(9 + n)
     H = rand_herm(rng, n)
-    c_ops = [0.2 * rand_c(rng, n, n) / np.sqrt(n)] + [np.sqrt(0.05) * np.diag(np.eye(n)[i]) for i in range(n)]
+    c_ops = ([0.2 * rand_c(rng, n, n) / np.sqrt(n)] if dense else []) \
+        + [np.sqrt(0.05) * np.diag(np.eye(n)[i]) for i in range(n)]
     rho0 = rand_dm(rng, n)
     rho, _, _ = _prop()(H, c_ops).run(rho0, 0.05, 500, renorm=True)
     r = rho[0].cpu().numpy()
@@ -232,9 +235,12 @@ def test_fp64_tiled_variant_matches_oracle(n, kd):
     assert np.max(np.abs(obs[0].cpu().numpy() - ref_obs)) < 1e-12
 
 
-@pytest.mark.parametrize("n,kd", [(8, 1), (30, 2), (48, 0), (64, 2), (80, 0), (96, 0)])
+@pytest.mark.parametrize("n,kd", [(8, 1), (30, 2), (48, 0), (64, 2), (80, 0), (96, 0),
+                                  (17, 0), (33, 0), (49, 0), (65, 0), (81, 0)])
 def test_every_launch_variant_matches_oracle(n, kd):
-    """Each resident launch shape (and the tiled tile choices) gives the same physics."""
+    """Each resident launch shape (and the tiled tile choices) gives the same physics.
+    n = 16 m + 1 without dense operators runs the bordered kernels (core on MFMA, last row
+    and column on VALU); the other sizes the padded tiles."""
     rng = np.random.default_rng(900 + n)
     H = rand_herm(rng, n)
     c_ops = [0.1 * rand_c(rng, n, n) / np.sqrt(n) for _ in range(kd)]
@@ -315,3 +321,24 @@ def test_tiled_trace_renormalisation_records_after_scaling(n, prec):
     err = np.max(np.abs(r - ref)) / np.max(np.abs(ref))
     print(f"tiled renorm n={n} {prec}: rel err {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("n,n_e", [(33, 1), (65, 1), (65, 2), (48, 1)])
+def test_qrt_per_operator_observables(n, n_e):
+    """One observable set per operator (eop_group = 1), as in the absorption ensemble: a
+    single observable is held in registers for the whole launch, several are read from
+    memory at every record; both against the oracle."""
+    rng = np.random.default_rng(300 + n + n_e)
+    S, G = 2, 3
+    Hs = np.stack([rand_herm(rng, n) for _ in range(S)])
+    L = [np.sqrt(0.03) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_c(rng, n, n) / n for _ in range(S * G)])
+    E = np.stack([np.stack([rand_c(rng, n, n) for _ in range(n_e)]) for _ in range(S * G)])
+    dt, nt = 0.05, 24
+    prop = _prop()(Hs, L, group=G)
+    rho, obs, _ = prop.run(X0, dt, nt, e_ops=torch.as_tensor(E), nout=3, eop_group=1)
+    obs = obs.cpu().numpy()
+    for b in range(S * G):
+        ref_rho, ref_obs, _ = ol.propagate(Hs[b // G], X0[b], L, dt, nt, list(E[b]), nout=3)
+        assert np.max(np.abs(obs[b] - ref_obs)) < 1e-6 * np.max(np.abs(ref_obs))
+        assert np.max(np.abs(rho[b].cpu().numpy() - ref_rho)) < 2e-7 * np.max(np.abs(ref_rho))
