@@ -33,6 +33,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(args):
+    """HBM bytes per launch of the default workload from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel calibration), or None when the
+    run does not match the profiled configuration."""
+    if (args.realisations, args.rk_per_step) != (4096, 256):
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "lindblad_rk4_resident_lattice65_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def flops_per_rho_step(n, k_dense=0):
     return 64.0 * n ** 3 * (1 + k_dense)
 
@@ -94,6 +111,8 @@ def main():
     ap.add_argument("--instances", type=int, default=2048, help="ragged: number of instances")
     ap.add_argument("--check-steps", type=int, default=100,
                     help="dense512: RK4 steps of the FP64 GPU reference comparison")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="launch-shape variant (default: autotuned on the workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a copy-like kernel (y = -x, float32) over MIB MiB three "
                          "times: the reference reading for the HBM byte counters")
@@ -144,6 +163,13 @@ def main():
     def bench_step(i):
         prop.step(X, dt, chunk, eops_t=E_t, eop_group=1, obs=obs, nout=1,
                   step_offset=i * chunk, rec_capacity=total_steps)
+
+    if args.variant is None:
+        from lime_amd import autotune
+        autotune.tune(prop, X, dt, steps=16, eops_t=E_t, eop_group=1, verbose=True, force=True)
+    else:
+        prop.variant = args.variant
+    log(f"[bench] launch variant {prop.variant}")
 
     for i in range(args.warmup):
         bench_step(i)
@@ -201,8 +227,10 @@ def main():
                    "flops_per_rho_step": fl},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA_TFLOPS,
-                     "traffic": None,
-                     "kernel": "lindblad_rk4_resident<5,13,2,4,false,true>",
+                     "traffic": profiled_traffic(args),
+                     "traffic_source": "profiles/lindblad_rk4_resident_lattice65_traffic.txt "
+                                       "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "kernel": f"lindblad_rk4_resident (bordered n=65, variant {prop.variant})",
                      "launch_ms_mean": float(np.mean(launch_ms))},
         "spectrum_peak": peak,
     }

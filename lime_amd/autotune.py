@@ -10,6 +10,7 @@ HIP events on the launch stream), keeps the fastest, and caches the choice per
 """
 import json
 import os
+import sys
 import threading
 
 import torch
@@ -53,12 +54,16 @@ def key_of(prop, n_eops=0):
             f":e{int(n_eops > 0)}:t{int(prop.force_tiled)}")
 
 
-def tune(prop, rho, dt, steps=4, n_eops=0, force=False, verbose=False):
+def tune(prop, rho, dt, steps=4, n_eops=0, force=False, verbose=False, eops_t=None, eop_group=None):
     """Pick the fastest launch variant for ``prop`` on operators shaped like ``rho``.
 
-    ``rho`` is not modified (a scratch copy is advanced).  Sets ``prop.variant`` and
-    returns (variant, {variant: ms}).
+    ``rho`` is not modified (a scratch copy is advanced).  With ``eops_t`` (the transposed
+    observables, as ``LindbladPropagator.step`` takes them) every timed step also records
+    the observables, as a recording run does.  Sets ``prop.variant`` and returns
+    (variant, {variant: ms}).
     """
+    if eops_t is not None:
+        n_eops = int(eops_t.shape[-3])
     with _lock:
         cache = _load()
         k = key_of(prop, n_eops)
@@ -72,24 +77,28 @@ def tune(prop, rho, dt, steps=4, n_eops=0, force=False, verbose=False):
     scratch = rho.clone()
     times = {}
     stream = torch.cuda.current_stream(prop.dev)
+    kw = {}
+    if eops_t is not None:
+        obs = torch.zeros((rho.shape[0], steps + 1, n_eops), dtype=torch.complex128, device=rho.device)
+        kw = dict(eops_t=eops_t, eop_group=eop_group, obs=obs, nout=1, rec_capacity=steps + 1)
     for v in range(nv):
         prop.variant = v
         try:
-            prop.step(scratch, dt, 1)                  # warm-up (and attribute set-up)
+            prop.step(scratch, dt, 1, **kw)                  # warm-up (and attribute set-up)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            prop.step(scratch, dt, steps)
+            prop.step(scratch, dt, steps, **kw)
             e1.record(stream)
             torch.cuda.synchronize(prop.dev)
             times[v] = e0.elapsed_time(e1) / steps
         except RuntimeError as err:                    # a variant this size cannot launch
             if verbose:
-                print(f"[tune] variant {v} skipped: {err}")
+                print(f"[tune] variant {v} skipped: {err}", file=sys.stderr)
     best = min(times, key=times.get)
     prop.variant = best
     if verbose:
         print(f"[tune] {key_of(prop, n_eops)}: " + ", ".join(f"v{v}={t:.3f}ms" for v, t in sorted(times.items()))
-              + f" -> v{best}")
+              + f" -> v{best}", file=sys.stderr)
     with _lock:
         _load()[key_of(prop, n_eops)] = {"variant": best, "ms": {str(v): t for v, t in times.items()}}
         _save()
