@@ -54,9 +54,11 @@ extern "C" {
  *   L_k = the dense (non-diagonal) jump operators; R_k = L_k unless `rops` is given
  *   (general sandwich terms, e.g. the Hilbert-space form of the Redfield tensor).
  *
- * n <= 96 runs the resident kernel (no workspace).  Larger n runs one launch per RK4
- * stage and needs `work` of lime_lindblad_workspace_bytes() bytes (caller-owned device
- * memory; the library never allocates).
+ * FP32 with n <= 96 (n <= 64 with dense operators) runs the resident kernel: one
+ * workgroup per operator for all steps, no workspace; n = 16m + 1 without dense operators
+ * uses its bordered layout.  Larger n, PREC_FP64, PREC_3XBF16 and FORCE_TILED run one
+ * launch per RK4 stage and need `work` of lime_lindblad_workspace_bytes() bytes
+ * (caller-owned device memory; the library never allocates).
  */
 typedef struct lime_lindblad_batch {
   int32_t batch;       /* number of propagated operators                               */
@@ -75,7 +77,7 @@ typedef struct lime_lindblad_batch {
   void* obs;           /* complex128 [batch][rec_capacity][n_eops] or NULL             */
   void* states;        /* complex128 [batch][rec_capacity][n][n] or NULL               */
   const void* rops;    /* complex64  [sets][n_dense][n][n] right factors, or NULL (= lops) */
-  void* work;          /* device workspace for n > 96 (see lime_lindblad_workspace_bytes) */
+  void* work;          /* device workspace of the per-stage kernels (lime_lindblad_workspace_bytes) */
   int64_t work_bytes;
 } lime_lindblad_batch;
 
@@ -108,7 +110,8 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
  *   y64 = copy of y in the operand precision (complex64, or complex128 with PREC_FP64)
  *   acc += c_acc V         (complex128, optional)
  * Complex scalars are passed as (re, im) pairs.  Uses d->heff/dmat/lops/rops/group and,
- * when n_dense > 0, d->work (>= batch * n_dense * n * n * 8 bytes) for L_k X.
+ * when n_dense > 0, d->work (>= batch * n_dense * n * n elements of the operand
+ * precision: 8 bytes, 16 with PREC_FP64) for L_k X.  Built for PREC_FP32 and PREC_FP64.
  * Replaces one call of lime's liouvillian (lime/oqs.py:706-713); with the adjoint operator
  * set (H_eff -> -H_eff^dag, D -> conj(D), (L, R) -> (L^dag, R^dag)) it applies f^dag.
  */
