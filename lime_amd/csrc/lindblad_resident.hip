@@ -98,7 +98,7 @@ __device__ __forceinline__ void lds_gemm(const float2* sA, const float2* sB, con
 // border outputs (row 16m and column 16m) are 65-term-or-shorter complex dot products on
 // VALU, each split over BG lanes.  The padded size of the plain kernel would spend
 // (m+1)^2 tiles on these outputs instead of m^2.
-template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD>
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD, bool DB>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(MINW)))
 lindblad_rk4_resident(ResidentArgs a) {
   constexpr int NP = NB * 16;
@@ -115,13 +115,17 @@ lindblad_rk4_resident(ResidentArgs a) {
   // per-lane caches of D and of a single observable (needs the register budget of one
   // tile per lane at <= 4 waves per SIMD, or of at most two waves per SIMD)
   constexpr bool CACHE = (BPW == 1 && MINW <= 4) || MINW <= 2;
+  // DB: double-buffered stage operand (when it fits the LDS; never with dense operators):
+  // the epilogue writes the next S into the other buffer, so a stage needs one barrier
+  static_assert(!(DB && DENSE), "the double buffer is built without dense operators");
 
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   float2* sH = reinterpret_cast<float2*>(lds_raw);
-  float2* sS = sH + NP * P;
-  // 4 spare rows after S (and after sT, sL) absorb the unused fragment prefetch past the
-  // last k step
-  float2* sT = sS + NP * P + 4 * P + 4;
+  float2* sS0 = sH + NP * P;
+  // 4 spare rows after each S buffer (and after sT, sL) absorb the unused fragment
+  // prefetch past the last k step
+  float2* sS1 = DB ? sS0 + NP * P + 4 * P + 4 : sS0;
+  float2* sT = sS1 + NP * P + 4 * P + 4;
   float2* sL = sT + (DENSE ? NP * P + 4 * P + 4 : 0);
   double2* sRed = reinterpret_cast<double2*>(sL + (DENSE ? NP * P + 4 * P + 4 : 0));
 
@@ -147,7 +151,10 @@ lindblad_rk4_resident(ResidentArgs a) {
       float2 v = make_float2(0.f, 0.f);
       if (i < n && j < n) v = H[(size_t)i * n + j];
       sH[i * P + j] = v;
-      if (BRD) sS[i * P + j] = make_float2(0.f, 0.f);
+      if (BRD) {
+        sS0[i * P + j] = make_float2(0.f, 0.f);
+        sS1[i * P + j] = make_float2(0.f, 0.f);
+      }
     }
   }
   if (BRD) __syncthreads();
@@ -198,12 +205,12 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (tval[u] && i < n && j < n) x = R[i * n + j];
         rho[u][v] = x;
         inc[u][v] = make_float2(0.f, 0.f);
-        if (tval[u]) sS[i * P + j] = make_float2((float)x.x, (float)x.y);
+        if (tval[u]) sS0[i * P + j] = make_float2((float)x.x, (float)x.y);
       }
     }
     if (bown) {
       rho_b = R[bi * n + bj];
-      if (bwrite) sS[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
+      if (bwrite) sS0[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
     }
   }
   // ---- register caches: D at the owned elements, the single observable's E^T
@@ -241,9 +248,12 @@ lindblad_rk4_resident(ResidentArgs a) {
   const bool renorm = (a.flags & 1) != 0;
   const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
 
+  int ph = 0;   // buffer holding the current stage operand
   for (int s = 0; s < a.nsteps; ++s) {
 #pragma unroll 1
     for (int st = 0; st < 4; ++st) {
+      const float2* sS = ph ? sS1 : sS0;      // read: operand of this stage
+      float2* sN = ph ? sS0 : sS1;            // written: operand of the next stage (== sS without DB)
       // Opaque copies of the lane indices: global addresses derived from them are
       // rebuilt inside the loop instead of being hoisted (and spilled) as 64-bit values.
       int olr = lr, olg = lg;
@@ -380,7 +390,7 @@ lindblad_rk4_resident(ResidentArgs a) {
           br -= dc_b.x * x.y + dc_b.y * x.x;
         }
       }
-      __syncthreads();  // every wave has finished reading S
+      if (!DB) __syncthreads();  // every wave has finished reading S
 
       const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
       const double cnext = (st == 2) ? dt : 0.5 * dt;
@@ -397,7 +407,7 @@ lindblad_rk4_resident(ResidentArgs a) {
             if (tval[u]) {
               const double sx = rho[u][v].x + cnext * (double)kr;
               const double sy = rho[u][v].y + cnext * (double)ki;
-              sS[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = make_float2((float)sx, (float)sy);
+              sN[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = make_float2((float)sx, (float)sy);
             }
           } else {
             rho[u][v].x += (double)ic.x;   // the state update itself is FP64
@@ -413,7 +423,7 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (st < 3) {
           inc_b = ic;
           if (bwrite)
-            sS[bi * P + bj] = make_float2((float)(rho_b.x + cnext * (double)kr), (float)(rho_b.y + cnext * (double)ki));
+            sN[bi * P + bj] = make_float2((float)(rho_b.x + cnext * (double)kr), (float)(rho_b.y + cnext * (double)ki));
         } else {
           rho_b.x += (double)ic.x;
           rho_b.y += (double)ic.y;
@@ -421,6 +431,7 @@ lindblad_rk4_resident(ResidentArgs a) {
       }
       if (st < 3) {
         __syncthreads();  // S of the next stage is complete
+        ph ^= (int)DB;
         continue;
       }
 
@@ -454,10 +465,10 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (!tval[u]) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          sS[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] =
+          sN[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] =
               make_float2((float)rho[u][v].x, (float)rho[u][v].y);
       }
-      if (BRD && bwrite) sS[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
+      if (BRD && bwrite) sN[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
 
       const int g = a.step_offset + s + 1;
       const bool rec_now = record_any && (g % a.nout) == 0 && (g / a.nout - 1) < a.rec_capacity;
@@ -513,6 +524,7 @@ lindblad_rk4_resident(ResidentArgs a) {
         }
       }
       __syncthreads();  // S of the next step (and the record partials) are complete
+      ph ^= (int)DB;
       if (rec_now && a.obs != nullptr) {
         const int rec = g / a.nout - 1;
         for (int e = tid; e < a.n_eops; e += T) {
@@ -574,9 +586,9 @@ constexpr Shape kBorderShapes[7][kMaxVariants] = {
     {{13, 2}, {9, 3}, {7, 4}, {0, 0}},
 };
 
-template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD>
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD, bool DB>
 hipError_t launch_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
-  auto k = lindblad_rk4_resident<NB, W, BPW, MINW, DENSE, REC, BRD>;
+  auto k = lindblad_rk4_resident<NB, W, BPW, MINW, DENSE, REC, BRD, DB>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -584,38 +596,59 @@ hipError_t launch_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <int NB, int W, int BPW, int MINW, bool DENSE, bool BRD>
-hipError_t launch_rec(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool BRD, bool DB>
+hipError_t launch_rec2(const ResidentArgs& a, size_t lds, hipStream_t stream) {
   const bool rec = a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
-  return rec ? launch_one<NB, W, BPW, MINW, DENSE, true, BRD>(a, lds, stream)
-             : launch_one<NB, W, BPW, MINW, DENSE, false, BRD>(a, lds, stream);
+  return rec ? launch_one<NB, W, BPW, MINW, DENSE, true, BRD, DB>(a, lds, stream)
+             : launch_one<NB, W, BPW, MINW, DENSE, false, BRD, DB>(a, lds, stream);
+}
+
+template <int NB, int W, int BPW, int MINW, bool DENSE, bool BRD>
+hipError_t launch_rec(const ResidentArgs& a, size_t lds, hipStream_t stream, bool db) {
+  if constexpr (!DENSE && NB <= 5) {
+    if (db) return launch_rec2<NB, W, BPW, MINW, DENSE, BRD, true>(a, lds, stream);
+  }
+  return launch_rec2<NB, W, BPW, MINW, DENSE, BRD, false>(a, lds, stream);
 }
 
 template <int NB, int W, int BPW>
-hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream, bool db) {
   static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
   constexpr int MINW = BPW == 1 ? 6 : BPW == 2 ? 4 : BPW == 3 ? 3 : 2;
   if constexpr (NB <= 4) {
-    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true, false>(a, lds, stream);
+    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true, false>(a, lds, stream, false);
   } else {
     if (a.n_dense > 0) return hipErrorInvalidValue;
   }
-  return launch_rec<NB, W, BPW, MINW, false, false>(a, lds, stream);
+  return launch_rec<NB, W, BPW, MINW, false, false>(a, lds, stream, db);
 }
 
 // Bordered shapes: one workgroup per CU at the larger sizes, so the register budget is
 // that of ceil(W / 4) waves per SIMD (at least 2 workgroups per CU below NB = 4).
 template <int NB, int W, int BPW>
-hipError_t launch_border(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+hipError_t launch_border(const ResidentArgs& a, size_t lds, hipStream_t stream, bool db) {
   static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
   constexpr int PER_SIMD = (W + 3) / 4;
   constexpr int MINW = NB <= 3 ? 2 * PER_SIMD : PER_SIMD;
   if (a.n_dense > 0 || a.n != 16 * (NB - 1) + 1) return hipErrorInvalidValue;
-  return launch_rec<NB, W, BPW, MINW, false, true>(a, lds, stream);
+  return launch_rec<NB, W, BPW, MINW, false, true>(a, lds, stream, db);
 }
 
 const Shape& shape_of(int nb, int variant, bool brd) {
   return brd ? kBorderShapes[nb][variant] : kShapes[nb][variant];
+}
+
+constexpr size_t kLdsMax = 160 * 1024;
+
+// LDS bytes of the layout: H_eff + nbuf operator buffers (NP x (NP+1) complex64 plus the
+// prefetch spare rows) + the FP64 reduction slots
+size_t lds_layout(int n, int variant, int n_dense, int n_eops, int nbuf) {
+  const int nb = cdiv(n, 16);
+  const int np = nb * 16, p = np + 1;
+  const size_t buf = (size_t)np * p + 4 * p + 4;
+  size_t bytes = (np * p + nbuf * buf) * sizeof(float2);
+  bytes += (size_t)(1 + n_eops) * shape_of(nb, variant, resident_bordered(n, n_dense)).w * sizeof(double2);
+  return bytes;
 }
 
 }  // namespace
@@ -633,54 +666,57 @@ int resident_variants(int n, int n_dense) {
   return c;
 }
 
+// H_eff + S + T + L with dense operators; without, H_eff + S double-buffered (nb <= 5)
+// when that fits, else a single S
 size_t resident_lds_bytes(int n, int variant, int n_dense, int n_eops) {
-  const int nb = cdiv(n, 16);
-  const int np = nb * 16, p = np + 1;
-  const size_t buf = (size_t)np * p + 4 * p + 4;   // one operator buffer + prefetch spare rows
-  size_t bytes = (np * p + (n_dense > 0 ? 3 : 1) * buf) * sizeof(float2);
-  bytes += (size_t)(1 + n_eops) * shape_of(nb, variant, resident_bordered(n, n_dense)).w * sizeof(double2);
-  return bytes;
+  if (n_dense > 0) return lds_layout(n, variant, n_dense, n_eops, 3);
+  return lds_layout(n, variant, n_dense, n_eops, resident_double_buffered(n, variant, n_dense, n_eops) ? 2 : 1);
+}
+
+bool resident_double_buffered(int n, int variant, int n_dense, int n_eops) {
+  return n_dense == 0 && cdiv(n, 16) <= 5 && lds_layout(n, variant, n_dense, n_eops, 2) <= kLdsMax;
 }
 
 hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t stream) {
   if (variant < 0 || variant >= resident_variants(a.n, a.n_dense)) return hipErrorInvalidValue;
   const size_t lds = resident_lds_bytes(a.n, variant, a.n_dense, a.n_eops);
+  const bool db = resident_double_buffered(a.n, variant, a.n_dense, a.n_eops);
   const int nb = cdiv(a.n, 16);
   if (resident_bordered(a.n, a.n_dense)) {
     switch (nb * 10 + variant) {
-      case 20: return launch_border<2, 4, 1>(a, lds, stream);
-      case 21: return launch_border<2, 2, 1>(a, lds, stream);
-      case 30: return launch_border<3, 4, 1>(a, lds, stream);
-      case 31: return launch_border<3, 8, 1>(a, lds, stream);
-      case 32: return launch_border<3, 2, 2>(a, lds, stream);
-      case 40: return launch_border<4, 9, 1>(a, lds, stream);
-      case 41: return launch_border<4, 5, 2>(a, lds, stream);
-      case 42: return launch_border<4, 3, 3>(a, lds, stream);
-      case 50: return launch_border<5, 16, 1>(a, lds, stream);
-      case 51: return launch_border<5, 8, 2>(a, lds, stream);
-      case 52: return launch_border<5, 4, 4>(a, lds, stream);
-      case 60: return launch_border<6, 13, 2>(a, lds, stream);
-      case 61: return launch_border<6, 9, 3>(a, lds, stream);
-      case 62: return launch_border<6, 7, 4>(a, lds, stream);
+      case 20: return launch_border<2, 4, 1>(a, lds, stream, db);
+      case 21: return launch_border<2, 2, 1>(a, lds, stream, db);
+      case 30: return launch_border<3, 4, 1>(a, lds, stream, db);
+      case 31: return launch_border<3, 8, 1>(a, lds, stream, db);
+      case 32: return launch_border<3, 2, 2>(a, lds, stream, db);
+      case 40: return launch_border<4, 9, 1>(a, lds, stream, db);
+      case 41: return launch_border<4, 5, 2>(a, lds, stream, db);
+      case 42: return launch_border<4, 3, 3>(a, lds, stream, db);
+      case 50: return launch_border<5, 16, 1>(a, lds, stream, db);
+      case 51: return launch_border<5, 8, 2>(a, lds, stream, db);
+      case 52: return launch_border<5, 4, 4>(a, lds, stream, db);
+      case 60: return launch_border<6, 13, 2>(a, lds, stream, db);
+      case 61: return launch_border<6, 9, 3>(a, lds, stream, db);
+      case 62: return launch_border<6, 7, 4>(a, lds, stream, db);
       default: return hipErrorInvalidValue;
     }
   }
   switch (nb * 10 + variant) {
-    case 10: return launch_shape<1, 1, 1>(a, lds, stream);
-    case 20: return launch_shape<2, 2, 2>(a, lds, stream);
-    case 21: return launch_shape<2, 4, 1>(a, lds, stream);
-    case 22: return launch_shape<2, 1, 4>(a, lds, stream);
-    case 30: return launch_shape<3, 3, 3>(a, lds, stream);
-    case 31: return launch_shape<3, 9, 1>(a, lds, stream);
-    case 32: return launch_shape<3, 5, 2>(a, lds, stream);
-    case 40: return launch_shape<4, 8, 2>(a, lds, stream);
-    case 41: return launch_shape<4, 4, 4>(a, lds, stream);
-    case 42: return launch_shape<4, 16, 1>(a, lds, stream);
-    case 43: return launch_shape<4, 6, 3>(a, lds, stream);
-    case 50: return launch_shape<5, 13, 2>(a, lds, stream);
-    case 51: return launch_shape<5, 9, 3>(a, lds, stream);
-    case 52: return launch_shape<5, 7, 4>(a, lds, stream);
-    case 60: return launch_shape<6, 12, 3>(a, lds, stream);
+    case 10: return launch_shape<1, 1, 1>(a, lds, stream, db);
+    case 20: return launch_shape<2, 2, 2>(a, lds, stream, db);
+    case 21: return launch_shape<2, 4, 1>(a, lds, stream, db);
+    case 22: return launch_shape<2, 1, 4>(a, lds, stream, db);
+    case 30: return launch_shape<3, 3, 3>(a, lds, stream, db);
+    case 31: return launch_shape<3, 9, 1>(a, lds, stream, db);
+    case 32: return launch_shape<3, 5, 2>(a, lds, stream, db);
+    case 40: return launch_shape<4, 8, 2>(a, lds, stream, db);
+    case 41: return launch_shape<4, 4, 4>(a, lds, stream, db);
+    case 42: return launch_shape<4, 16, 1>(a, lds, stream, db);
+    case 43: return launch_shape<4, 6, 3>(a, lds, stream, db);
+    case 50: return launch_shape<5, 13, 2>(a, lds, stream, db);
+    case 51: return launch_shape<5, 9, 3>(a, lds, stream, db);
+    case 52: return launch_shape<5, 7, 4>(a, lds, stream, db);
+    case 60: return launch_shape<6, 12, 3>(a, lds, stream, db);
     default: return hipErrorInvalidValue;
   }
 }

@@ -25,6 +25,9 @@ int resident_nb_max(int n_dense);
 bool resident_bordered(int n, int n_dense);
 int resident_variants(int n, int n_dense);
 size_t resident_lds_bytes(int n, int variant, int n_dense, int n_eops);
+// does this launch use the double-buffered stage operand (no dense operators, n <= 80,
+// and the layout fits the LDS)?
+bool resident_double_buffered(int n, int variant, int n_dense, int n_eops);
 hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t stream);
 
 }  // namespace lime
