@@ -112,9 +112,16 @@ lindblad_rk4_resident(ResidentArgs a) {
   constexpr int BG = (T >= 4 * NBP) ? 4 : (T >= 2 * NBP) ? 2 : 1;
   static_assert(!BRD || T >= NBP, "every border output needs an owner");
   static_assert(!(BRD && DENSE), "bordered kernels are built without dense jump operators");
-  // per-lane caches of D and of a single observable (needs the register budget of one
-  // tile per lane at <= 4 waves per SIMD, or of at most two waves per SIMD)
+  // per-lane cache of D (needs the register budget of one tile per lane at <= 4 waves per
+  // SIMD, or of at most two waves per SIMD); of a single observable only at <= 2 waves per
+  // SIMD (at 4 the registers go to the split-sign accumulators, which pay more)
   constexpr bool CACHE = (BPW == 1 && MINW <= 4) || MINW <= 2;
+  constexpr bool ECACHE = CACHE && MINW <= 2;
+  // split-sign accumulation: the positive and the negative real products of C go to
+  // separate accumulators (subtracted once per stage), so the MFMA stream carries no
+  // sign-flip VALU instructions (+8 % at n = 65).  Built where the register budget
+  // (512 / MINW VGPRs) covers ~56 VGPRs per tile + 72 of fixed state.
+  constexpr bool SPLIT = 512 / MINW >= 56 * BPW + 72;
   // DB: double-buffered stage operand (when it fits the LDS; never with dense operators):
   // the epilogue writes the next S into the other buffer, so a stage needs one barrier
   static_assert(!(DB && DENSE), "the double buffer is built without dense operators");
@@ -217,8 +224,8 @@ lindblad_rk4_resident(ResidentArgs a) {
   const bool has_d = a.dmat != nullptr;
   float2 dc[CACHE ? BPW : 1][4];
   float2 dc_b = make_float2(0.f, 0.f);
-  const bool ecached = CACHE && REC && a.obs != nullptr && a.n_eops == 1;
-  double2 ec[CACHE && REC ? BPW : 1][4];
+  const bool ecached = ECACHE && REC && a.obs != nullptr && a.n_eops == 1;
+  double2 ec[ECACHE && REC ? BPW : 1][4];
   double2 ec_b = make_double2(0.0, 0.0);
   if constexpr (CACHE) {
     const float2* Dm = a.dmat + (size_t)set * nn;
@@ -231,7 +238,7 @@ lindblad_rk4_resident(ResidentArgs a) {
         const int i = trow[u] + 4 * lg + v;
         const bool in = tval[u] && i < n && j < n;
         dc[u][v] = (has_d && in) ? Dm[i * n + j] : make_float2(0.f, 0.f);
-        if constexpr (REC) ec[u][v] = (ecached && in) ? E[i * n + j] : make_double2(0.0, 0.0);
+        if constexpr (ECACHE && REC) ec[u][v] = (ecached && in) ? E[i * n + j] : make_double2(0.0, 0.0);
       }
     }
     if (bown) {
@@ -285,6 +292,13 @@ lindblad_rk4_resident(ResidentArgs a) {
           z[u] = sS[oR[u]];
           h[u] = sH[oHc[u]];
         }
+        // SPLIT: cr, ci collect the positive products of Re C, Im C; dr, di the negative
+        floatx4 dr[SPLIT ? BPW : 1], di[SPLIT ? BPW : 1];
+#pragma unroll
+        for (int u = 0; u < (SPLIT ? BPW : 1); ++u) {
+          dr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+          di[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
 #pragma unroll 2
         for (int k0 = 0; k0 < ke; k0 += 4) {
           float2 xn[BPW], yn[BPW], zn[BPW], hn[BPW];
@@ -298,14 +312,27 @@ lindblad_rk4_resident(ResidentArgs a) {
           }
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
-            cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
-            ci[u] = mfma_f32(x[u].x, y[u].y, ci[u]);
-            cr[u] = mfma_f32(-x[u].y, y[u].y, cr[u]);
-            ci[u] = mfma_f32(x[u].y, y[u].x, ci[u]);
-            cr[u] = mfma_f32(-z[u].x, h[u].x, cr[u]);
-            ci[u] = mfma_f32(z[u].x, h[u].y, ci[u]);
-            cr[u] = mfma_f32(-z[u].y, h[u].y, cr[u]);
-            ci[u] = mfma_f32(-z[u].y, h[u].x, ci[u]);
+            if constexpr (SPLIT) {
+              // Re C = Hx Sx - (Hy Sy + Sx Hx' + Sy Hy'),  Im C = Hx Sy + Hy Sx + Sx Hy' - Sy Hx'
+              // (H' = H[j][k] of the S H^dag product)
+              cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
+              ci[u] = mfma_f32(x[u].x, y[u].y, ci[u]);
+              dr[u] = mfma_f32(x[u].y, y[u].y, dr[u]);
+              di[u] = mfma_f32(z[u].y, h[u].x, di[u]);
+              ci[u] = mfma_f32(x[u].y, y[u].x, ci[u]);
+              dr[u] = mfma_f32(z[u].x, h[u].x, dr[u]);
+              ci[u] = mfma_f32(z[u].x, h[u].y, ci[u]);
+              dr[u] = mfma_f32(z[u].y, h[u].y, dr[u]);
+            } else {
+              cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
+              ci[u] = mfma_f32(x[u].x, y[u].y, ci[u]);
+              cr[u] = mfma_f32(-x[u].y, y[u].y, cr[u]);
+              ci[u] = mfma_f32(x[u].y, y[u].x, ci[u]);
+              cr[u] = mfma_f32(-z[u].x, h[u].x, cr[u]);
+              ci[u] = mfma_f32(z[u].x, h[u].y, ci[u]);
+              cr[u] = mfma_f32(-z[u].y, h[u].y, cr[u]);
+              ci[u] = mfma_f32(-z[u].y, h[u].x, ci[u]);
+            }
           }
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
@@ -313,6 +340,13 @@ lindblad_rk4_resident(ResidentArgs a) {
             y[u] = yn[u];
             z[u] = zn[u];
             h[u] = hn[u];
+          }
+        }
+        if constexpr (SPLIT) {
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            cr[u] -= dr[u];
+            ci[u] -= di[u];
           }
         }
       }
@@ -487,7 +521,7 @@ lindblad_rk4_resident(ResidentArgs a) {
                 const int i = trow[u] + 4 * lg + v;
                 if (tval[u] && i < n && j < n) {
                   double2 ev;
-                  if constexpr (CACHE && REC) {
+                  if constexpr (ECACHE && REC) {
                     ev = ecached ? ec[u][v] : E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
                   } else {
                     ev = E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
