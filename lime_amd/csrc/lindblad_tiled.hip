@@ -121,6 +121,14 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
   using C = typename CT<R>::C;
   constexpr int BW = TILE / 32;
   constexpr int NR = TILE * KP / 256;
+  // 16x16 blocks entirely outside the operator hold zeros: their MFMAs are skipped
+  // (wave-uniform), which frees the matrix pipe for the other workgroups of the CU
+  bool rlive[BW], clive[BW];
+#pragma unroll
+  for (int p = 0; p < BW; ++p) {
+    rlive[p] = row0 + wr * (TILE / 2) + p * 16 < n;
+    clive[p] = col0 + wc * (TILE / 2) + p * 16 < n;
+  }
   C ra[NR], rb[NR];
   load_rows<TILE>(A, n, row0, 0, ra, tid);
   if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, 0, col0, rb, tid);
@@ -138,6 +146,7 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
     }
 #pragma unroll
     for (int kk = 0; kk < KP; kk += 4) {
+      if (k0 + kk >= n) break;                // the last panel: k-steps past n are zeros
       C x[BW], y[BW];
 #pragma unroll
       for (int p = 0; p < BW; ++p) {
@@ -149,6 +158,7 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
       for (int p = 0; p < BW; ++p)
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
+          if (!(rlive[p] && clive[q])) continue;
           const C a = x[p], h = y[q];
           if (MODE == PLAIN) {
             cr[p][q] = CT<R>::mfma(a.x, h.x, cr[p][q]);
@@ -232,6 +242,12 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
                                               int lr, int lg) {
   constexpr int BW = TILE / 32;
   constexpr int NR = TILE * KP / 256;
+  bool rlive[BW], clive[BW];
+#pragma unroll
+  for (int p = 0; p < BW; ++p) {
+    rlive[p] = row0 + wr * (TILE / 2) + p * 16 < n;
+    clive[p] = col0 + wc * (TILE / 2) + p * 16 < n;
+  }
   unsigned short* sA = reinterpret_cast<unsigned short*>(smem);
   unsigned short* sBr = sA + TILE * BP;
   unsigned short* sBi = sBr + TILE * BP;
@@ -256,6 +272,7 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
+      if (m == 1 && k0 + 8 >= n) continue;    // main slots of k0+8..k0+15: zeros past n
       u16x8 x[BW], yr[BW], yi[BW];
 #pragma unroll
       for (int p = 0; p < BW; ++p) {
@@ -268,6 +285,7 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
       for (int p = 0; p < BW; ++p)
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
+          if (!(rlive[p] && clive[q])) continue;
           const bf16x8 a = __builtin_bit_cast(bf16x8, x[p]);
           cr[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, yr[q]), cr[p][q], 0, 0, 0);
           ci[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, yi[q]), ci[p][q], 0, 0, 0);
