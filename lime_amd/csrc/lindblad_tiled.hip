@@ -28,8 +28,12 @@ namespace lime {
 
 namespace {
 
-constexpr int KP = 16;          // K-panel width (complex elements)
-constexpr int KPP = KP + 1;     // LDS pitch of [row][k] panels (odd: conflict-free fragments)
+constexpr int KP = 16;          // K-panel width (complex elements) of the 3xBF16 staging
+#ifndef LIME_TILED_KW
+#define LIME_TILED_KW 16
+#endif
+constexpr int KW = LIME_TILED_KW;   // K-panel width of the FP32 / FP64 staging
+constexpr int KWP = KW + 1;         // LDS pitch of [row][k] panels (odd: conflict-free fragments)
 
 template <typename R> struct CT;
 template <> struct CT<float> {
@@ -55,14 +59,14 @@ template <> struct CT<double> {
 //   I_AH  : C += i A M^dag
 enum Mode { PLAIN = 0, NEG_AH = 1, I_AH = 2 };
 
-// Load a TILE x KP panel of a row-major n x n matrix: rows r0.., cols k0..
-template <int TILE, typename C>
+// Load a TILE x PW panel of a row-major n x n matrix: rows r0.., cols k0..
+template <int TILE, int PW, typename C>
 __device__ __forceinline__ void load_rows(const C* __restrict__ M, int n, int r0, int k0,
-                                          C (&reg)[TILE * KP / 256], int tid) {
+                                          C (&reg)[TILE * PW / 256], int tid) {
 #pragma unroll
-  for (int q = 0; q < TILE * KP / 256; ++q) {
+  for (int q = 0; q < TILE * PW / 256; ++q) {
     const int e = tid + q * 256;
-    const int r = e / KP, c = e % KP;
+    const int r = e / PW, c = e % PW;
     const int gr = r0 + r, gc = k0 + c;
     C v;
     v.x = 0;
@@ -72,21 +76,21 @@ __device__ __forceinline__ void load_rows(const C* __restrict__ M, int n, int r0
   }
 }
 
-template <int TILE, typename C>
-__device__ __forceinline__ void store_rows(C* sM, const C (&reg)[TILE * KP / 256], int tid) {
+template <int TILE, int PW, typename C>
+__device__ __forceinline__ void store_rows(C* sM, const C (&reg)[TILE * PW / 256], int tid) {
 #pragma unroll
-  for (int q = 0; q < TILE * KP / 256; ++q) {
+  for (int q = 0; q < TILE * PW / 256; ++q) {
     const int e = tid + q * 256;
-    sM[(e / KP) * KPP + e % KP] = reg[q];
+    sM[(e / PW) * (PW + 1) + e % PW] = reg[q];
   }
 }
 
-// Load a KP x TILE panel: rows k0.. (the k index), cols c0..
-template <int TILE, typename C>
+// Load a PW x TILE panel: rows k0.. (the k index), cols c0..
+template <int TILE, int PW, typename C>
 __device__ __forceinline__ void load_cols(const C* __restrict__ M, int n, int k0, int c0,
-                                          C (&reg)[TILE * KP / 256], int tid) {
+                                          C (&reg)[TILE * PW / 256], int tid) {
 #pragma unroll
-  for (int q = 0; q < TILE * KP / 256; ++q) {
+  for (int q = 0; q < TILE * PW / 256; ++q) {
     const int e = tid + q * 256;
     const int r = e / TILE, c = e % TILE;
     const int gr = k0 + r, gc = c0 + c;
@@ -98,17 +102,17 @@ __device__ __forceinline__ void load_cols(const C* __restrict__ M, int n, int k0
   }
 }
 
-template <int TILE, typename C>
-__device__ __forceinline__ void store_cols(C* sM, const C (&reg)[TILE * KP / 256], int tid) {
+template <int TILE, int PW, typename C>
+__device__ __forceinline__ void store_cols(C* sM, const C (&reg)[TILE * PW / 256], int tid) {
 #pragma unroll
-  for (int q = 0; q < TILE * KP / 256; ++q) {
+  for (int q = 0; q < TILE * PW / 256; ++q) {
     const int e = tid + q * 256;
     sM[(e / TILE) * (TILE + 1) + e % TILE] = reg[q];
   }
 }
 
 template <int TILE>
-constexpr int smem_b() { return KP * (TILE + 1) > TILE * KPP ? KP * (TILE + 1) : TILE * KPP; }
+constexpr int smem_b() { return KW * (TILE + 1) > TILE * KWP ? KW * (TILE + 1) : TILE * KWP; }
 
 // Accumulate one product term into the wave's MFMA blocks.
 template <int TILE, int MODE, typename R>
@@ -120,7 +124,7 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
                                           int wc, int lr, int lg) {
   using C = typename CT<R>::C;
   constexpr int BW = TILE / 32;
-  constexpr int NR = TILE * KP / 256;
+  constexpr int NR = TILE * KW / 256;
   // 16x16 blocks entirely outside the operator hold zeros: their MFMAs are skipped
   // (wave-uniform), which frees the matrix pipe for the other workgroups of the CU
   bool rlive[BW], clive[BW];
@@ -130,29 +134,29 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
     clive[p] = col0 + wc * (TILE / 2) + p * 16 < n;
   }
   C ra[NR], rb[NR];
-  load_rows<TILE>(A, n, row0, 0, ra, tid);
-  if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, 0, col0, rb, tid);
-  else load_rows<TILE>(Bsrc, n, col0, 0, rb, tid);
-  for (int k0 = 0; k0 < n; k0 += KP) {
+  load_rows<TILE, KW>(A, n, row0, 0, ra, tid);
+  if (MODE == PLAIN) load_cols<TILE, KW>(Bsrc, n, 0, col0, rb, tid);
+  else load_rows<TILE, KW>(Bsrc, n, col0, 0, rb, tid);
+  for (int k0 = 0; k0 < n; k0 += KW) {
     __syncthreads();                          // previous panel fully consumed
-    store_rows<TILE>(sA, ra, tid);
-    if (MODE == PLAIN) store_cols<TILE>(sB, rb, tid);
-    else store_rows<TILE>(sB, rb, tid);
+    store_rows<TILE, KW>(sA, ra, tid);
+    if (MODE == PLAIN) store_cols<TILE, KW>(sB, rb, tid);
+    else store_rows<TILE, KW>(sB, rb, tid);
     __syncthreads();
-    if (k0 + KP < n) {                        // prefetch the next panel into registers
-      load_rows<TILE>(A, n, row0, k0 + KP, ra, tid);
-      if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, k0 + KP, col0, rb, tid);
-      else load_rows<TILE>(Bsrc, n, col0, k0 + KP, rb, tid);
+    if (k0 + KW < n) {                        // prefetch the next panel into registers
+      load_rows<TILE, KW>(A, n, row0, k0 + KW, ra, tid);
+      if (MODE == PLAIN) load_cols<TILE, KW>(Bsrc, n, k0 + KW, col0, rb, tid);
+      else load_rows<TILE, KW>(Bsrc, n, col0, k0 + KW, rb, tid);
     }
 #pragma unroll
-    for (int kk = 0; kk < KP; kk += 4) {
+    for (int kk = 0; kk < KW; kk += 4) {
       if (k0 + kk >= n) break;                // the last panel: k-steps past n are zeros
       C x[BW], y[BW];
 #pragma unroll
       for (int p = 0; p < BW; ++p) {
-        x[p] = sA[(wr * (TILE / 2) + p * 16 + lr) * KPP + kk + lg];
+        x[p] = sA[(wr * (TILE / 2) + p * 16 + lr) * KWP + kk + lg];
         if (MODE == PLAIN) y[p] = sB[(kk + lg) * (TILE + 1) + wc * (TILE / 2) + p * 16 + lr];
-        else y[p] = sB[(wc * (TILE / 2) + p * 16 + lr) * KPP + kk + lg];
+        else y[p] = sB[(wc * (TILE / 2) + p * 16 + lr) * KWP + kk + lg];
       }
 #pragma unroll
       for (int p = 0; p < BW; ++p)
@@ -252,9 +256,9 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
   unsigned short* sBr = sA + TILE * BP;
   unsigned short* sBi = sBr + TILE * BP;
   float2 ra[NR], rb[NR];
-  load_rows<TILE>(A, n, row0, 0, ra, tid);
-  if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, 0, col0, rb, tid);
-  else load_rows<TILE>(Bsrc, n, col0, 0, rb, tid);
+  load_rows<TILE, KP>(A, n, row0, 0, ra, tid);
+  if (MODE == PLAIN) load_cols<TILE, KP>(Bsrc, n, 0, col0, rb, tid);
+  else load_rows<TILE, KP>(Bsrc, n, col0, 0, rb, tid);
   for (int k0 = 0; k0 < n; k0 += KP) {
     __syncthreads();                          // previous panel fully consumed
 #pragma unroll
@@ -266,9 +270,9 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
     }
     __syncthreads();
     if (k0 + KP < n) {                        // prefetch the next panel into registers
-      load_rows<TILE>(A, n, row0, k0 + KP, ra, tid);
-      if (MODE == PLAIN) load_cols<TILE>(Bsrc, n, k0 + KP, col0, rb, tid);
-      else load_rows<TILE>(Bsrc, n, col0, k0 + KP, rb, tid);
+      load_rows<TILE, KP>(A, n, row0, k0 + KP, ra, tid);
+      if (MODE == PLAIN) load_cols<TILE, KP>(Bsrc, n, k0 + KP, col0, rb, tid);
+      else load_rows<TILE, KP>(Bsrc, n, col0, k0 + KP, rb, tid);
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
@@ -297,7 +301,7 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
 // LDS bytes of a GEMM-staging area.
 template <int TILE, typename R, int P>
 constexpr int smem_bytes() {
-  return P == 1 ? 3 * TILE * BP * 2 : (TILE * KPP + smem_b<TILE>()) * (int)sizeof(typename CT<R>::C);
+  return P == 1 ? 3 * TILE * BP * 2 : (TILE * KWP + smem_b<TILE>()) * (int)sizeof(typename CT<R>::C);
 }
 
 // One product term in the selected arithmetic.
@@ -312,11 +316,26 @@ __device__ __forceinline__ void term(const typename CT<R>::C* __restrict__ A, co
   } else {
     using C = typename CT<R>::C;
     C* sA = reinterpret_cast<C*>(smem);
-    gemm_term<TILE, MODE, R>(A, B, n, row0, col0, sA, sA + TILE * KPP, cr, ci, tid, wr, wc, lr, lg);
+    gemm_term<TILE, MODE, R>(A, B, n, row0, col0, sA, sA + TILE * KWP, cr, ci, tid, wr, wc, lr, lg);
   }
 }
 
 __device__ __forceinline__ int tiles_per_dim(int n, int tile) { return (n + tile - 1) / tile; }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx b runs on XCD b % 8),
+// each with its own L2.  Renumber them so that XCD x runs a contiguous range of logical
+// ids: the tiles of one operator (consecutive logical ids) then share one L2 instead of
+// every XCD fetching the operator's panels.  A bijection for any grid size.
+__device__ __forceinline__ int xcd_block() {
+#ifdef LIME_NO_XCD_REMAP
+  return blockIdx.x;
+#else
+  constexpr int X = 8;
+  const int g = gridDim.x, b = blockIdx.x;
+  const int per = g / X, rem = g % X, x = b % X;
+  return x * per + (x < rem ? x : rem) + b / X;
+#endif
+}
 
 // Common per-workgroup geometry.
 struct Geo {
@@ -367,8 +386,9 @@ __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
   using V4 = typename CT<R>::V4;
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   const int per = tiles_per_dim(a.n, TILE) * tiles_per_dim(a.n, TILE);
-  const int bk = blockIdx.x / per;            // = b * n_dense + k
-  const Geo g = geometry<TILE>(a.n, blockIdx.x % per);
+  const int blk = xcd_block();
+  const int bk = blk / per;                   // = b * n_dense + k
+  const Geo g = geometry<TILE>(a.n, blk % per);
   const int b = bk / a.n_dense, k = bk % a.n_dense;
   const int set = b / a.group;
   const size_t nn = (size_t)a.n * a.n;
@@ -403,8 +423,9 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
   __shared__ double2 sRed[4];
   const int n = a.n;
   const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
-  const int b = blockIdx.x / per;
-  const Geo g = geometry<TILE>(n, blockIdx.x % per);
+  const int blk = xcd_block();
+  const int b = blk / per;
+  const Geo g = geometry<TILE>(n, blk % per);
   const int set = b / a.group;
   const size_t nn = (size_t)n * n;
   constexpr int BW = TILE / 32;
@@ -512,8 +533,9 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   const int n = a.n;
   const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
-  const int b = blockIdx.x / per;
-  const Geo g = geometry<TILE>(n, blockIdx.x % per);
+  const int blk = xcd_block();
+  const int b = blk / per;
+  const Geo g = geometry<TILE>(n, blk % per);
   const int set = b / a.group;
   const size_t nn = (size_t)n * n;
   constexpr int BW = TILE / 32;
