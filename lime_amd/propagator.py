@@ -40,6 +40,17 @@ def as_c128(x, dev):
     return t.contiguous()
 
 
+def _check(t, what, shape, dtype, dev):
+    """Every buffer handed to the C ABI is indexed by the kernels from the descriptor's
+    sizes alone: its shape, dtype, layout and device are checked here, on the host."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch tensor")
+    if t.dtype != dtype or not t.is_contiguous() or t.device != dev:
+        raise ValueError(f"{what} must be a contiguous {dtype} tensor on {dev}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
 def _stack_sets(x, nsets, n, dev, what):
     t = as_c128(x, dev)
     if t.dim() == 2:
@@ -146,17 +157,36 @@ class LindbladPropagator:
         self.variant = 0
         self._work = None
         self._lib = _lib.load()
+        if precision not in _PREC_FLAGS:
+            raise ValueError(f"unknown precision {precision!r}; expected one of {sorted(_PREC_FLAGS)}")
         cdt = torch.complex128 if precision == "fp64" else torch.complex64
         heff = as_c128(heff, self.dev)
+        if heff.dim() == 2:
+            heff = heff.unsqueeze(0)
+        if heff.dim() != 3 or heff.shape[-1] != heff.shape[-2]:
+            raise ValueError(f"heff must be (N, N) or (S, N, N), got {tuple(heff.shape)}")
         self.nsets, self.n = heff.shape[0], heff.shape[-1]
         self.group = int(group)
+        if self.group < 1:
+            raise ValueError("group must be >= 1")
         self.heff_c128 = heff
         self.heff = heff.to(cdt).contiguous()
-        self.dmat = as_c128(dmat, self.dev).to(cdt).contiguous() if dmat is not None else None
-        self.lops = as_c128(lops, self.dev).to(cdt).contiguous() if lops is not None else None
-        self.rops = as_c128(rops, self.dev).to(cdt).contiguous() if rops is not None else None
+        self.dmat = _stack_sets(dmat, self.nsets, self.n, self.dev, "dmat").to(cdt).contiguous() \
+            if dmat is not None else None
+        self.lops = self._op_stack(lops, "lops", cdt)
+        self.rops = self._op_stack(rops, "rops", cdt)
+        if self.rops is not None and (self.lops is None or self.rops.shape != self.lops.shape):
+            raise ValueError("rops needs lops of the same shape")
         self.n_dense = 0 if self.lops is None else self.lops.shape[1]
         return self
+
+    def _op_stack(self, x, what, cdt):
+        if x is None:
+            return None
+        t = as_c128(x, self.dev)
+        if t.dim() != 4 or t.shape[0] != self.nsets or t.shape[-2:] != (self.n, self.n):
+            raise ValueError(f"{what} has shape {tuple(t.shape)}, expected ({self.nsets}, K, {self.n}, {self.n})")
+        return t.to(cdt).contiguous()
 
     @property
     def batch(self):
@@ -197,13 +227,14 @@ class LindbladPropagator:
               stream=None):
         """One generator application V = f(x) (x in the operand precision) with the epilogue
         y = c_lx V + c_z z, y64 = y in the operand precision, acc += c_acc V (all optional)."""
-        B = x64.shape[0]
-        op_dtype = torch.complex128 if self.precision == "fp64" else torch.complex64
-        if x64.dtype != op_dtype or not x64.is_contiguous():
-            raise ValueError(f"x must be a contiguous {op_dtype} tensor (operand precision)")
-        for t in (z, y, acc):
-            if t is not None and (t.dtype != torch.complex128 or not t.is_contiguous()):
-                raise ValueError("z, y, acc must be contiguous complex128 tensors")
+        B, n = self.batch, self.n
+        op_dtype = self.operand_dtype
+        _check(x64, "x (operand precision)", (B, n, n), op_dtype, self.dev)
+        for name, t in (("z", z), ("y", y), ("acc", acc)):
+            if t is not None:
+                _check(t, name, (B, n, n), torch.complex128, self.dev)
+        if y64 is not None:
+            _check(y64, "y64 (operand precision)", (B, n, n), op_dtype, self.dev)
         d = self._descriptor(B)
         if self.n_dense > 0:
             need = B * self.n_dense * self.n * self.n * (16 if self.precision == "fp64" else 8)
@@ -230,15 +261,27 @@ class LindbladPropagator:
     def step(self, rho, dt, nsteps, *, eops_t=None, eop_group=None, obs=None, states=None,
              nout=1, step_offset=0, rec_capacity=0, renorm=False, stream=None):
         """Advance ``rho`` (B, N, N) complex128 in place by ``nsteps`` RK4 steps (async)."""
-        n, B = self.n, rho.shape[0]
-        if rho.dtype != torch.complex128 or not rho.is_contiguous() or rho.device != self.dev:
-            raise ValueError("rho must be a contiguous complex128 tensor on the GPU")
-        if rho.shape != (B, n, n) or B != self.batch:
-            raise ValueError(f"rho has shape {tuple(rho.shape)}, expected ({self.batch}, {n}, {n})")
+        n, B = self.n, self.batch
+        _check(rho, "rho", (B, n, n), torch.complex128, self.dev)
+        eop_group = int(eop_group or B)
+        if eop_group < 1:
+            raise ValueError("eop_group must be >= 1")
+        n_e = 0
+        if eops_t is not None:
+            if eops_t.dim() != 4:
+                raise ValueError("eops_t must be (E_sets, n_e, N, N)")
+            n_e = int(eops_t.shape[1])
+            _check(eops_t, "eops_t", (-(-B // eop_group), n_e, n, n), torch.complex128, self.dev)
+        if obs is not None:
+            if eops_t is None:
+                raise ValueError("obs needs eops_t")
+            _check(obs, "obs", (B, int(rec_capacity), n_e), torch.complex128, self.dev)
+        if states is not None:
+            _check(states, "states", (B, int(rec_capacity), n, n), torch.complex128, self.dev)
         d = _lib.LindbladBatch()
         d.batch, d.n, d.group, d.n_dense = B, n, self.group, self.n_dense
-        d.n_eops = 0 if eops_t is None else int(eops_t.shape[-3])
-        d.eop_group = int(eop_group or B)
+        d.n_eops = n_e
+        d.eop_group = eop_group
         d.flags = _PREC_FLAGS[self.precision] | (_lib.LIME_F_RENORM_TRACE if renorm else 0) \
             | (_lib.LIME_F_FORCE_TILED if self.force_tiled else 0)
         d.variant = self.variant

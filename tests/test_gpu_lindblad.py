@@ -342,3 +342,37 @@ def test_qrt_per_operator_observables(n, n_e):
         ref_rho, ref_obs, _ = ol.propagate(Hs[b // G], X0[b], L, dt, nt, list(E[b]), nout=3)
         assert np.max(np.abs(obs[b] - ref_obs)) < 1e-6 * np.max(np.abs(ref_obs))
         assert np.max(np.abs(rho[b].cpu().numpy() - ref_rho)) < 2e-7 * np.max(np.abs(ref_rho))
+
+
+def test_host_checks_reject_buffers_the_kernel_would_overrun():
+    """Every buffer handed to the C ABI is sized from the descriptor alone, so a buffer of
+    the wrong shape, dtype or layout is refused on the host before any launch."""
+    rng = np.random.default_rng(3)
+    n = 10
+    H = np.stack([rand_herm(rng, n) for _ in range(2)])
+    prop = _prop()(H, [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)], group=3)   # batch 6
+    dev = torch.device("cuda")
+    rho = torch.zeros((6, n, n), dtype=torch.complex128, device=dev)
+    E = torch.zeros((6, 1, n, n), dtype=torch.complex128, device=dev)
+    obs = torch.zeros((6, 4, 1), dtype=torch.complex128, device=dev)
+    with pytest.raises(ValueError):
+        prop.step(rho[:5].contiguous(), 0.1, 1)                                  # batch
+    with pytest.raises(ValueError):
+        prop.step(rho.to(torch.complex64), 0.1, 1)                               # dtype
+    with pytest.raises(ValueError):
+        prop.step(rho, 0.1, 1, eops_t=E[:3].contiguous(), eop_group=1, obs=obs, rec_capacity=4)
+    with pytest.raises(ValueError):
+        prop.step(rho, 0.1, 1, eops_t=E, eop_group=1, obs=obs[:, :2].contiguous(), rec_capacity=4)
+    with pytest.raises(ValueError):
+        prop.step(rho, 0.1, 1, eops_t=E.transpose(-1, -2), eop_group=1, obs=obs, rec_capacity=4)
+    with pytest.raises(ValueError):
+        prop.step(rho, 0.1, 1, states=torch.zeros((6, 2, n, n), dtype=torch.complex128, device=dev),
+                  rec_capacity=4)
+    x = torch.zeros((6, n, n), dtype=torch.complex64, device=dev)
+    with pytest.raises(ValueError):
+        prop.apply(x[:4].contiguous())
+    with pytest.raises(ValueError):
+        prop.apply(x, y=torch.zeros((6, n, n), dtype=torch.complex64, device=dev))
+    # the well-formed call still runs
+    prop.step(rho, 0.1, 4, eops_t=E, eop_group=1, obs=obs, nout=1, rec_capacity=4)
+    torch.cuda.synchronize()
