@@ -60,18 +60,35 @@ def _run(H, rho0, c_ops, dt, Nt, e_ops, nout=1, store_states=False, renorm=False
 
 
 # ------------------------------------------------------------------ right-hand side
+def _apply_generator(rho, H, c_ops):
+    """One application of the Lindblad generator on the GPU (FP64-MFMA generator kernel,
+    ``lime_lindblad_apply``).  rho: (N, N) or (B, N, N); NumPy in -> NumPy out."""
+    dev = device()
+    as_numpy = not isinstance(rho, torch.Tensor)
+    r = as_c128(rho, dev)
+    single = r.dim() == 2
+    if single:
+        r = r.unsqueeze(0)
+    Ht = as_c128(H, dev)
+    group = r.shape[0] if Ht.dim() == 2 else 1
+    prop = LindbladPropagator(Ht, c_ops or [], group=group, precision="fp64")
+    if prop.batch != r.shape[0]:
+        raise ValueError(f"rho has {r.shape[0]} operators, H has {prop.nsets} sets")
+    y = torch.empty_like(r)
+    prop.apply(r.contiguous(), y=y, c_lx=1.0)
+    out = y[0] if single else y
+    return _np(out) if as_numpy else out
+
+
 def lindbladian(l, rho):
-    """L rho L^dag - 1/2 {L^dag L, rho} (oqs.py:716-723)."""
-    ld = phys.dag(l)
-    return l @ (rho @ ld) - 0.5 * phys.anticomm(ld @ l, rho)
+    """L rho L^dag - 1/2 {L^dag L, rho} (oqs.py:716-723), evaluated on the GPU."""
+    n = (rho.shape[-1])
+    return _apply_generator(rho, np.zeros((n, n), dtype=np.complex128), [l])
 
 
 def liouvillian(rho, H, c_ops):
-    """-i[H, rho] + sum_k D[L_k] rho (oqs.py:706-713)."""
-    rhs = -1j * phys.comm(H, rho)
-    for c in c_ops or []:
-        rhs = rhs + lindbladian(c, rho)
-    return rhs
+    """-i[H, rho] + sum_k D[L_k] rho (oqs.py:706-713), evaluated on the GPU."""
+    return _apply_generator(rho, H, c_ops)
 
 
 # ------------------------------------------------------------------ drivers
