@@ -204,20 +204,25 @@ class Redfield_solver:
         return -1j * self.propagator(t, method="EOM" if method == "EOM" else "SOS")
 
     def expect(self, rho0, e_ops):
+        """<<e|U(t)|rho0>> in the eigenbasis for each e_op (oqs.py:225-252), on the GPU."""
+        dev = device()
         V = self.evecs
-        rho_eb = so.dm2vec(V.conj().T @ _dense(rho0) @ V)
-        e_eb = [V.conj().T @ _dense(e) @ V for e in e_ops]
-        rho_t = np.tensordot(self.U, rho_eb, axes=([1], [0]))          # (N^2, Nt)
-        return np.stack([so.dm2vec(e) @ rho_t for e in e_eb], axis=1)
+        rho_eb = as_c128(so.dm2vec(V.conj().T @ _dense(rho0) @ V), dev)
+        E = as_c128(np.stack([so.dm2vec(V.conj().T @ _dense(e) @ V) for e in e_ops]), dev)
+        U = as_c128(self.U, dev)                                        # (N^2, N^2, Nt)
+        rho_t = torch.einsum("abk,b->ak", U, rho_eb)                    # (N^2, Nt)
+        return _dense((E @ rho_t).transpose(0, 1))
 
     def correlation_2op_1t(self, rho0, a, b, tau):
         if self.G is None:
             self.propagator(tau)
-        idm = so.dm2vec(np.identity(self.dim))
+        dev = device()
+        idm = as_c128(so.dm2vec(np.identity(self.dim)), dev)
         r0 = _dense(rho0)
         r0 = so.dm2vec(r0) if r0.ndim == 2 else r0
-        x = _dense(b) @ r0
-        return idm @ (_dense(a) @ np.tensordot(self.G, x, axes=([1], [0])))
+        x = as_c128(_dense(b) @ r0, dev)
+        Gx = torch.einsum("abk,b->ak", as_c128(self.G, dev), x)          # (N^2, Nt)
+        return _dense(idm @ (as_c128(_dense(a), dev) @ Gx))
 
     def correlation_4op_3t(self, rho0, oplist, signature, tau):
         """<<I| a G(t3) b G(t2) c G(t1) d |rho0>> as an (Nt, Nt, Nt) array (oqs.py:277-366)."""

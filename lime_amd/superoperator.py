@@ -111,8 +111,11 @@ def trace(rho):
 
 
 def resolvent(omega, L):
-    L = _dense(L)
-    return np.linalg.inv(omega * np.identity(L.shape[0]) - L)
+    """(omega - L)^-1 (superoperator.py resolvent), inverted on the GPU; NumPy out."""
+    dev = device()
+    Lt = torch.as_tensor(np.asarray(_dense(L), dtype=np.complex128), device=dev)
+    eye = torch.eye(Lt.shape[0], dtype=torch.complex128, device=dev)
+    return torch.linalg.inv(omega * eye - Lt).cpu().numpy()
 
 
 class Lindblad_solver:

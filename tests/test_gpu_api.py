@@ -240,3 +240,11 @@ def test_liouvillian_and_lindbladian_on_gpu_match_lime_sequence():
     Y = oqs.liouvillian(X, H, c_ops)
     for b in range(3):
         assert np.max(np.abs(Y[b].cpu().numpy() - ol.liouvillian_ref(X[b].cpu().numpy(), H, c_ops))) < 1e-12
+
+
+def test_resolvent_on_gpu():
+    from lime_amd import superoperator as so
+    rng, H, c_ops = model(8, n=4)
+    L = osu.liouvillian(H, c_ops)
+    R = so.resolvent(0.3 + 0.1j, L)
+    assert np.max(np.abs(R @ ((0.3 + 0.1j) * np.eye(16) - L) - np.eye(16))) < 1e-12

@@ -96,3 +96,16 @@ def test_liouville_redfield_function_and_propagators():
     got = s.correlation_4op_3t(rho0, ops, sig, t[:5])
     ref = orf.response_4op_3t(R.toarray(), rho0, sup, t[:5])
     assert got.shape == (5, 5, 5) and np.max(np.abs(got - ref)) < 1e-9
+    # expect / correlation_2op_1t (oqs.py:225-274) contract U(t) on the GPU
+    U = s.propagator(t, "SOS")
+    V = evecs
+    vec = lambda m: m.reshape(-1)
+    e = rand_herm(rng, n)
+    r_eb = vec(V.conj().T @ rho0 @ V)
+    ref_e = np.array([vec(V.conj().T @ e @ V) @ (U_ref[:, :, k] @ r_eb) for k in range(len(t))])
+    assert np.max(np.abs(s.expect(rho0, [e])[:, 0] - ref_e)) < 1e-9
+    A = osu.operator_to_superoperator(rand_herm(rng, n), "left")
+    B = osu.operator_to_superoperator(rand_herm(rng, n), "right")
+    idm = vec(np.eye(n))
+    ref_c = np.array([idm @ (A @ (-1j * U_ref[:, :, k]) @ (B @ vec(rho0))) for k in range(len(t))])
+    assert np.max(np.abs(s.correlation_2op_1t(rho0, A, B, t) - ref_c)) < 1e-9
