@@ -66,6 +66,7 @@ class LindbladBatch(ctypes.Structure):
         ("rops", ctypes.c_void_p),
         ("work", ctypes.c_void_p),
         ("work_bytes", ctypes.c_int64),
+        ("stage_states", ctypes.c_void_p),
     ]
 
 

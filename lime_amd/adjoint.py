@@ -15,7 +15,8 @@ checkpoint X_{cK} (default K = 64).
 Backward, for a real loss with torch's gradient G_k = dL/dC_k^* convention
 (dL = Re sum_k conj(G_k) dC_k): the source of step k is g_k = G_k mu^dag, and with the RK4
 stages s_1 = X, s_2 = X + h/2 f(s_1), s_3 = X + h/2 f(s_2), s_4 = X + h f(s_3) of step k
-(recomputed from the checkpoint of their segment) and lambda' = lambda_{k+1}:
+(recomputed from the checkpoint of their segment on the per-stage kernels, which record
+s_2, s_3, s_4 in FP64 as the step computes them) and lambda' = lambda_{k+1}:
     kappa_4 = h/6 lambda',               sigma_4 = f^dag(kappa_4)
     kappa_3 = h/3 lambda' + h sigma_4,   sigma_3 = f^dag(kappa_3)
     kappa_2 = h/3 lambda' + h/2 sigma_3, sigma_2 = f^dag(kappa_2)
@@ -78,6 +79,8 @@ class ChainQRT:
             from . import autotune
             autotune.tune(self.prop, self.x0, self.dt)
         self.adj = self.prop.adjoint()
+        # the backward recompute records the stage operands on the per-stage kernels
+        self.rec_prop = self.prop.tiled()
         self.mu = dipole_operator(torch.as_tensor(mu_sites, device=self.dev), self.n)
         self.x0[:, :, 0] = self.mu[:, 0]
 
@@ -103,17 +106,22 @@ class ChainQRT:
         return obs[:, :, 0]
 
     def _segment_states(self, c0, length):
-        """X_{c0}, ..., X_{c0+length-1} recomputed from the checkpoint at step c0."""
+        """Recompute steps c0 .. c0+length-1 from the checkpoint at step c0.
+
+        Returns X (B, length, N, N) with X[:, j] = X_{c0+j} and the FP64 stage operands
+        (B, length, 3, N, N) of the step leaving X_{c0+j}: X + h/2 k1, X + h/2 k2, X + h k3
+        -- the values the forward step used, recorded by the per-stage kernels."""
         B, n = self.B, self.n
         Xc = self.x0.clone() if c0 == 0 else self.checkpoints[:, c0 // self.K - 1].clone().contiguous()
         out = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
         out[:, 0] = Xc
+        stages = torch.empty((B, length, 3, n, n), dtype=torch.complex128, device=self.dev)
+        st = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
+        self.rec_prop.step(Xc, self.dt, length, states=st, stage_states=stages, nout=1,
+                           step_offset=0, rec_capacity=length)
         if length > 1:
-            st = torch.empty((B, length - 1, n, n), dtype=torch.complex128, device=self.dev)
-            self.prop.step(Xc, self.dt, length - 1, states=st, nout=1, step_offset=0,
-                           rec_capacity=length - 1)
-            out[:, 1:] = st
-        return out
+            out[:, 1:] = st[:, :length - 1]
+        return out, stages
 
     def backward(self, G):
         """G (B, Nt) complex: torch's dL/dC^*.  Returns (g_eps (B,ns), g_J (B,ns-1), g_gamma (B,))."""
@@ -128,8 +136,8 @@ class ChainQRT:
         c128 = lambda: torch.empty((B, n, n), dtype=torch.complex128, device=dev)
         opd = self.prop.operand_dtype
         c64 = lambda: torch.empty((B, n, n), dtype=opd, device=dev)
-        s2, s3, s4, k3, k2, k1 = c128(), c128(), c128(), c128(), c128(), c128()
-        s2l, s3l, s4l, k3l, k2l, k1l = c64(), c64(), c64(), c64(), c64(), c64()
+        k3, k2, k1 = c128(), c128(), c128()
+        k3l, k2l, k1l = c64(), c64(), c64()
         lib = self.prop._lib
         from . import _lib as L
         stream = torch.cuda.current_stream(dev)
@@ -137,15 +145,12 @@ class ChainQRT:
         for seg in range(nseg - 1, -1, -1):
             c0 = seg * self.K
             length = min(self.K, self.nt - c0)
-            Xs = self._segment_states(c0, length)
+            Xs, stages = self._segment_states(c0, length)
             for k in range(c0 + length - 1, c0 - 1, -1):
                 # lambda' = lambda_{k+1}: add the source of record k+1 (C_{k+1} = Tr[mu X_{k+1}])
                 lam.add_(G[:, k].reshape(B, 1, 1) * mu_h)
                 X = Xs[:, k - c0].contiguous()
-                Xl = X.to(opd).contiguous()
-                self.prop.apply(Xl, z=X, y=s2, y64=s2l, c_lx=h / 2, c_z=1.0)
-                self.prop.apply(s2l, z=X, y=s3, y64=s3l, c_lx=h / 2, c_z=1.0)
-                self.prop.apply(s3l, z=X, y=s4, y64=s4l, c_lx=h, c_z=1.0)
+                s2, s3, s4 = (stages[:, k - c0, i].contiguous() for i in range(3))
                 k4 = (h / 6.0) * lam
                 k4l = k4.to(opd).contiguous()
                 acc = lam.clone()

@@ -90,7 +90,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   if (d->n_eops < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n_eops < 0");
   if (d->obs != nullptr && (d->n_eops == 0 || d->eops_t == nullptr))
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: obs needs n_eops > 0 and eops_t");
-  if ((d->obs != nullptr || d->states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))
+  if ((d->obs != nullptr || d->states != nullptr || d->stage_states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: recording needs nout >= 1 and rec_capacity >= 0");
   if (p->step_offset < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: step_offset < 0");
   if ((d->flags & LIME_F_PREC_MASK) == 0x30)
@@ -98,6 +98,9 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   const hipStream_t s = static_cast<hipStream_t>(stream);
 
   if (use_resident(d)) {
+    if (d->stage_states != nullptr)
+      return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: stage operands are recorded by the per-stage "
+                                        "kernels only (set LIME_F_FORCE_TILED)");
     lime::ResidentArgs a;
     a.batch = d->batch;
     a.n = d->n;
@@ -148,7 +151,8 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
   a.rho = static_cast<double2*>(d->rho);
   a.obs = static_cast<double2*>(d->obs);
   a.states = static_cast<double2*>(d->states);
-  const int nout = (d->obs || d->states) ? p->nout : 0;
+  a.stage_rec = static_cast<double2*>(d->stage_states);
+  const int nout = (d->obs || d->states || d->stage_states) ? p->nout : 0;
   const lime::TiledPrec prec = is_fp64(d) ? lime::TiledPrec::FP64
                               : is_split_bf16(d) ? lime::TiledPrec::SPLIT_BF16 : lime::TiledPrec::FP32;
   hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s, tiled_tile(d), prec);
@@ -198,6 +202,8 @@ int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const voi
 int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const void* s, double w,
                           void* out_a, void* out_b, void* out_u, void* stream) {
   if (batch < 0 || n < 1) return fail(LIME_ERR_ARG, "lime_adjoint_contract: bad batch/n");
+  if (lime::contract_lds_bytes(n) > 160 * 1024)
+    return fail(LIME_ERR_UNSUPPORTED, "lime_adjoint_contract: n > 2048 is not built");
   if (batch == 0) return LIME_OK;
   if (!kappa || !s || !out_a || !out_b || (n > 1 && !out_u))
     return fail(LIME_ERR_ARG, "lime_adjoint_contract: null pointer");

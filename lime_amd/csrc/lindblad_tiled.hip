@@ -478,7 +478,10 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
         double2 r = rho[o];
         if (st < 3) {
           inc[o] = ic;
-          Sn[o] = CT<R>::make(r.x + cnext * (double)kr, r.y + cnext * (double)ki);
+          const double sx = r.x + cnext * (double)kr, sy = r.y + cnext * (double)ki;
+          Sn[o] = CT<R>::make(sx, sy);
+          if (a.stage_rec && a.rec >= 0)        // FP64 stage operand of this recorded step
+            a.stage_rec[(((size_t)b * a.rec_capacity + a.rec) * 3 + st) * nn + o] = make_double2(sx, sy);
         } else {
           r.x += (double)ic.x;
           r.y += (double)ic.y;
@@ -590,50 +593,85 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
 //   out_b[j] += w (P_jj - (sum_m P_jm + sum_m P_mj) / 2)    -> d/d(dephasing rate of j)
 //   out_u[j] += w (sum_m conj(k_jm) s_j+1,m + conj(k_j+1,m) s_jm
 //                  - sum_m conj(k_m,j+1) s_mj + conj(k_mj) s_m,j+1)   -> d/dH_j,j+1 (times -i)
-// One workgroup per operator; thread t owns index j = t, t + 256, ...
+// One workgroup per operator, every global read coalesced: a column sweep (thread per
+// column, rows in order) gives the column sums, the diagonal and the column-pair terms; a
+// row sweep (wave per row, lanes along the row, FP64 wave reductions) the row sums and
+// row-pair terms; the two meet in LDS (5 n complex doubles).
 __global__ void __launch_bounds__(256) adjoint_contract(ContractArgs c) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char craw[];
   const int b = blockIdx.x, n = c.n;
   const size_t nn = (size_t)n * n;
   const double2* K = c.kappa + (size_t)b * nn;
   const double2* S = c.s + (size_t)b * nn;
+  double2* cs = reinterpret_cast<double2*>(craw);   // column sums
+  double2* cp = cs + n;                              // column-pair terms
+  double2* dg = cp + n;                              // diagonal
+  double2* rs = dg + n;                              // row sums
+  double2* rp = rs + n;                              // row-pair terms
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double rr = 0, ri = 0, cr_ = 0, ci_ = 0, ur = 0, ui = 0;
+    double sr = 0, si = 0, pr = 0, pi = 0;
     for (int m = 0; m < n; ++m) {
-      const double2 k1 = K[(size_t)j * n + m], s1 = S[(size_t)j * n + m];   // row j
-      rr += k1.x * s1.x + k1.y * s1.y;
-      ri += k1.x * s1.y - k1.y * s1.x;
-      const double2 k2 = K[(size_t)m * n + j], s2 = S[(size_t)m * n + j];   // column j
-      cr_ += k2.x * s2.x + k2.y * s2.y;
-      ci_ += k2.x * s2.y - k2.y * s2.x;
+      const double2 k = K[(size_t)m * n + j], x = S[(size_t)m * n + j];
+      sr += k.x * x.x + k.y * x.y;
+      si += k.x * x.y - k.y * x.x;
       if (j + 1 < n) {
-        const double2 s3 = S[(size_t)(j + 1) * n + m];                     // conj(k_jm) s_j+1,m
-        const double2 k4 = K[(size_t)(j + 1) * n + m];                     // conj(k_j+1,m) s_jm
-        const double2 k5 = K[(size_t)m * n + j + 1];                       // conj(k_m,j+1) s_mj
-        const double2 s6 = S[(size_t)m * n + j + 1];                       // conj(k_mj) s_m,j+1
-        ur += (k1.x * s3.x + k1.y * s3.y) + (k4.x * s1.x + k4.y * s1.y)
-            - (k5.x * s2.x + k5.y * s2.y) - (k2.x * s6.x + k2.y * s6.y);
-        ui += (k1.x * s3.y - k1.y * s3.x) + (k4.x * s1.y - k4.y * s1.x)
-            - (k5.x * s2.y - k5.y * s2.x) - (k2.x * s6.y - k2.y * s6.x);
+        const double2 k1 = K[(size_t)m * n + j + 1], x1 = S[(size_t)m * n + j + 1];
+        pr += (k1.x * x.x + k1.y * x.y) + (k.x * x1.x + k.y * x1.y);
+        pi += (k1.x * x.y - k1.y * x.x) + (k.x * x1.y - k.y * x1.x);
       }
     }
     const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
-    const double dr = kd.x * sd.x + kd.y * sd.y, di = kd.x * sd.y - kd.y * sd.x;
+    cs[j] = make_double2(sr, si);
+    cp[j] = make_double2(pr, pi);
+    dg[j] = make_double2(kd.x * sd.x + kd.y * sd.y, kd.x * sd.y - kd.y * sd.x);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int r = wave; r < n; r += nw) {
+    double sr = 0, si = 0, pr = 0, pi = 0;
+    for (int m = lane; m < n; m += 64) {
+      const double2 k = K[(size_t)r * n + m], x = S[(size_t)r * n + m];
+      sr += k.x * x.x + k.y * x.y;
+      si += k.x * x.y - k.y * x.x;
+      if (r + 1 < n) {
+        const double2 k1 = K[(size_t)(r + 1) * n + m], x1 = S[(size_t)(r + 1) * n + m];
+        pr += (k.x * x1.x + k.y * x1.y) + (k1.x * x.x + k1.y * x.y);
+        pi += (k.x * x1.y - k.y * x1.x) + (k1.x * x.y - k1.y * x.x);
+      }
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    pr = wave_sum(pr);
+    pi = wave_sum(pi);
+    if (lane == 0) {
+      rs[r] = make_double2(sr, si);
+      rp[r] = make_double2(pr, pi);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double2 R = rs[j], Cc = cs[j], D = dg[j];
     double2* A = c.out_a + (size_t)b * n + j;
-    A->x += c.weight * (rr - cr_);
-    A->y += c.weight * (ri - ci_);
+    A->x += c.weight * (R.x - Cc.x);
+    A->y += c.weight * (R.y - Cc.y);
     double2* B = c.out_b + (size_t)b * n + j;
-    B->x += c.weight * (dr - 0.5 * (rr + cr_));
-    B->y += c.weight * (di - 0.5 * (ri + ci_));
+    B->x += c.weight * (D.x - 0.5 * (R.x + Cc.x));
+    B->y += c.weight * (D.y - 0.5 * (R.y + Cc.y));
     if (j + 1 < n) {
       double2* U = c.out_u + (size_t)b * (n - 1) + j;
-      U->x += c.weight * ur;
-      U->y += c.weight * ui;
+      U->x += c.weight * (rp[j].x - cp[j].x);
+      U->y += c.weight * (rp[j].y - cp[j].y);
     }
   }
 }
 
+size_t contract_lds_bytes(int n) { return (size_t)5 * n * sizeof(double2); }
+
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
-  hipLaunchKernelGGL(adjoint_contract, dim3(c.batch), dim3(256), 0, stream, c);
+  const size_t lds = contract_lds_bytes(c.n);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(adjoint_contract),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(adjoint_contract, dim3(c.batch), dim3(256), lds, stream, c);
   return hipGetLastError();
 }
 
@@ -753,7 +791,7 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
   for (int s = 0; s < nsteps; ++s) {
     const int gstep = step_offset + s + 1;
     a.rec = -1;
-    if (nout > 0 && (a.obs || a.states) && gstep % nout == 0 && gstep / nout - 1 < a.rec_capacity)
+    if (nout > 0 && (a.obs || a.states || a.stage_rec) && gstep % nout == 0 && gstep / nout - 1 < a.rec_capacity)
       a.rec = gstep / nout - 1;
     // stage 3 writes S = rho into s0 (stages alternate s0 -> s1 -> s0 -> s1 -> s0)
     hipError_t e = (tile == 64) ? run_step<64, R, P>(a, stream, s0, s1) : run_step<32, R, P>(a, stream, s0, s1);

@@ -25,6 +25,7 @@ struct TiledArgs {
   void* inc;
   void* tbuf;
   double* tscale;          // per-operator 1 / Re Tr rho of the trace renormalisation
+  double2* stage_rec;      // [batch][rec_capacity][3][n][n] FP64 stage operands, or NULL
   // generator application (lindblad_apply_tiled): V = L(s_cur);
   //   y = c_lx V + c_z z (complex128), y64 = copy of y in the operand precision,
   //   acc += c_acc V
@@ -54,6 +55,7 @@ struct ContractArgs {
   double2* out_b;         // [batch][n]   += w (diag - (rowsum + colsum)/2)
   double2* out_u;         // [batch][n-1] += w (nearest-neighbour coupling contraction)
 };
+size_t contract_lds_bytes(int n);
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream);
 
 }  // namespace lime

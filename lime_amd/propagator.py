@@ -192,6 +192,16 @@ class LindbladPropagator:
     def batch(self):
         return self.nsets * self.group
 
+    def tiled(self):
+        """The same operator batch on the per-stage kernels (shares the device operators)."""
+        t = object.__new__(LindbladPropagator)
+        t.__dict__.update(self.__dict__)
+        t._work = None
+        resident = (not self.force_tiled and self.precision == "fp32" and self.n <= self.max_n())
+        t.force_tiled = True
+        t.variant = 0 if resident else self.variant     # a tuned tile choice carries over
+        return t
+
     def adjoint(self):
         """The propagator of the adjoint generator f^dag (same batch layout):
         H_eff -> -H_eff^dag, D -> conj(D), (L, R) -> (L^dag, R^dag)."""
@@ -259,8 +269,12 @@ class LindbladPropagator:
         return self._lib.lime_lindblad_max_n(self.n_dense, _PREC_FLAGS[self.precision])
 
     def step(self, rho, dt, nsteps, *, eops_t=None, eop_group=None, obs=None, states=None,
-             nout=1, step_offset=0, rec_capacity=0, renorm=False, stream=None):
-        """Advance ``rho`` (B, N, N) complex128 in place by ``nsteps`` RK4 steps (async)."""
+             nout=1, step_offset=0, rec_capacity=0, renorm=False, stream=None, stage_states=None):
+        """Advance ``rho`` (B, N, N) complex128 in place by ``nsteps`` RK4 steps (async).
+
+        ``stage_states`` (B, rec_capacity, 3, N, N) complex128 receives the FP64 stage
+        operands X + h/2 k1, X + h/2 k2, X + h k3 of every recorded step (per-stage
+        kernels: the propagator must be ``force_tiled`` for n <= 96)."""
         n, B = self.n, self.batch
         _check(rho, "rho", (B, n, n), torch.complex128, self.dev)
         eop_group = int(eop_group or B)
@@ -278,6 +292,8 @@ class LindbladPropagator:
             _check(obs, "obs", (B, int(rec_capacity), n_e), torch.complex128, self.dev)
         if states is not None:
             _check(states, "states", (B, int(rec_capacity), n, n), torch.complex128, self.dev)
+        if stage_states is not None:
+            _check(stage_states, "stage_states", (B, int(rec_capacity), 3, n, n), torch.complex128, self.dev)
         d = _lib.LindbladBatch()
         d.batch, d.n, d.group, d.n_dense = B, n, self.group, self.n_dense
         d.n_eops = n_e
@@ -293,6 +309,7 @@ class LindbladPropagator:
         d.obs = obs.data_ptr() if obs is not None else None
         d.states = states.data_ptr() if states is not None else None
         d.rops = self.rops.data_ptr() if self.rops is not None else None
+        d.stage_states = stage_states.data_ptr() if stage_states is not None else None
         need = int(self._lib.lime_lindblad_workspace_bytes(ctypes.byref(d)))
         if need > 0:
             if self._work is None or self._work.numel() < need:
