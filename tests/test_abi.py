@@ -67,6 +67,9 @@ def test_argument_validation_without_gpu(lib):
     assert _call(lib, obs=0x3000) == _lib.LIME_ERR_ARG         # obs without e_ops
     assert _call(lib, flags=_lib.LIME_F_PREC_FP64) == _lib.LIME_ERR_ARG   # tiled: needs workspace
     assert _call(lib, flags=0x30) == _lib.LIME_ERR_UNSUPPORTED
+    # stage operands are recorded by the per-stage kernels only
+    assert _call(lib, stage_states=0x5000) == _lib.LIME_ERR_UNSUPPORTED
+    assert b"FORCE_TILED" in lib.lime_last_error()
     # 3xBF16 runs on the tiled kernels at every n: it needs a workspace too
     assert _call(lib, flags=_lib.LIME_F_PREC_3XBF16) == _lib.LIME_ERR_ARG
     assert b"workspace" in lib.lime_last_error()
