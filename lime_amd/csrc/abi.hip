@@ -203,7 +203,7 @@ int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const voi
                           void* out_a, void* out_b, void* out_u, void* stream) {
   if (batch < 0 || n < 1) return fail(LIME_ERR_ARG, "lime_adjoint_contract: bad batch/n");
   if (lime::contract_lds_bytes(n) > 160 * 1024)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_adjoint_contract: n > 2048 is not built");
+    return fail(LIME_ERR_UNSUPPORTED, "lime_adjoint_contract: n > 930 is not built");
   if (batch == 0) return LIME_OK;
   if (!kappa || !s || !out_a || !out_b || (n > 1 && !out_u))
     return fail(LIME_ERR_ARG, "lime_adjoint_contract: null pointer");

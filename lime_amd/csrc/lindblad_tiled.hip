@@ -593,24 +593,30 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
 //   out_b[j] += w (P_jj - (sum_m P_jm + sum_m P_mj) / 2)    -> d/d(dephasing rate of j)
 //   out_u[j] += w (sum_m conj(k_jm) s_j+1,m + conj(k_j+1,m) s_jm
 //                  - sum_m conj(k_m,j+1) s_mj + conj(k_mj) s_m,j+1)   -> d/dH_j,j+1 (times -i)
-// One workgroup per operator, every global read coalesced: a column sweep (thread per
-// column, rows in order) gives the column sums, the diagonal and the column-pair terms; a
-// row sweep (wave per row, lanes along the row, FP64 wave reductions) the row sums and
-// row-pair terms; the two meet in LDS (5 n complex doubles).
-__global__ void __launch_bounds__(256) adjoint_contract(ContractArgs c) {
+// One 1024-thread workgroup per operator, every global read coalesced: a column sweep
+// (thread per column, the rows split into CQ quarters whose partial sums meet in LDS in a
+// fixed order) gives the column sums, the diagonal and the column-pair terms; a row sweep
+// (wave per row, lanes along the row, FP64 wave reductions) the row sums and row-pair
+// terms.  Deterministic: no atomics.
+constexpr int CT_THREADS = 1024;
+constexpr int CQ = CT_THREADS / 256;
+
+__global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   extern __shared__ __attribute__((aligned(16))) unsigned char craw[];
   const int b = blockIdx.x, n = c.n;
   const size_t nn = (size_t)n * n;
   const double2* K = c.kappa + (size_t)b * nn;
   const double2* S = c.s + (size_t)b * nn;
-  double2* cs = reinterpret_cast<double2*>(craw);   // column sums
-  double2* cp = cs + n;                              // column-pair terms
-  double2* dg = cp + n;                              // diagonal
+  double2* cs = reinterpret_cast<double2*>(craw);   // [CQ][n] column-sum partials
+  double2* cp = cs + CQ * n;                         // [CQ][n] column-pair partials
+  double2* dg = cp + CQ * n;                         // diagonal
   double2* rs = dg + n;                              // row sums
   double2* rp = rs + n;                              // row-pair terms
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  const int q = threadIdx.x / 256, tq = threadIdx.x % 256;
+  const int rows = (n + CQ - 1) / CQ, m0 = q * rows, m1 = min(n, m0 + rows);
+  for (int j = tq; j < n; j += 256) {
     double sr = 0, si = 0, pr = 0, pi = 0;
-    for (int m = 0; m < n; ++m) {
+    for (int m = m0; m < m1; ++m) {
       const double2 k = K[(size_t)m * n + j], x = S[(size_t)m * n + j];
       sr += k.x * x.x + k.y * x.y;
       si += k.x * x.y - k.y * x.x;
@@ -620,10 +626,12 @@ __global__ void __launch_bounds__(256) adjoint_contract(ContractArgs c) {
         pi += (k1.x * x.y - k1.y * x.x) + (k.x * x1.y - k.y * x1.x);
       }
     }
-    const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
-    cs[j] = make_double2(sr, si);
-    cp[j] = make_double2(pr, pi);
-    dg[j] = make_double2(kd.x * sd.x + kd.y * sd.y, kd.x * sd.y - kd.y * sd.x);
+    cs[q * n + j] = make_double2(sr, si);
+    cp[q * n + j] = make_double2(pr, pi);
+    if (q == 0) {
+      const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
+      dg[j] = make_double2(kd.x * sd.x + kd.y * sd.y, kd.x * sd.y - kd.y * sd.x);
+    }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int r = wave; r < n; r += nw) {
@@ -649,7 +657,14 @@ __global__ void __launch_bounds__(256) adjoint_contract(ContractArgs c) {
   }
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const double2 R = rs[j], Cc = cs[j], D = dg[j];
+    double2 Cc = make_double2(0.0, 0.0), Cp = make_double2(0.0, 0.0);
+    for (int x = 0; x < CQ; ++x) {
+      Cc.x += cs[x * n + j].x;
+      Cc.y += cs[x * n + j].y;
+      Cp.x += cp[x * n + j].x;
+      Cp.y += cp[x * n + j].y;
+    }
+    const double2 R = rs[j], D = dg[j];
     double2* A = c.out_a + (size_t)b * n + j;
     A->x += c.weight * (R.x - Cc.x);
     A->y += c.weight * (R.y - Cc.y);
@@ -658,20 +673,20 @@ __global__ void __launch_bounds__(256) adjoint_contract(ContractArgs c) {
     B->y += c.weight * (D.y - 0.5 * (R.y + Cc.y));
     if (j + 1 < n) {
       double2* U = c.out_u + (size_t)b * (n - 1) + j;
-      U->x += c.weight * (rp[j].x - cp[j].x);
-      U->y += c.weight * (rp[j].y - cp[j].y);
+      U->x += c.weight * (rp[j].x - Cp.x);
+      U->y += c.weight * (rp[j].y - Cp.y);
     }
   }
 }
 
-size_t contract_lds_bytes(int n) { return (size_t)5 * n * sizeof(double2); }
+size_t contract_lds_bytes(int n) { return (size_t)(2 * CQ + 3) * n * sizeof(double2); }
 
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
   const size_t lds = contract_lds_bytes(c.n);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(adjoint_contract),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(adjoint_contract, dim3(c.batch), dim3(256), lds, stream, c);
+  hipLaunchKernelGGL(adjoint_contract, dim3(c.batch), dim3(CT_THREADS), lds, stream, c);
   return hipGetLastError();
 }
 
