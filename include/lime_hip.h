@@ -79,9 +79,10 @@ typedef struct lime_lindblad_batch {
   const void* rops;    /* complex64  [sets][n_dense][n][n] right factors, or NULL (= lops) */
   void* work;          /* device workspace of the per-stage kernels (lime_lindblad_workspace_bytes) */
   int64_t work_bytes;
-  void* stage_states;  /* complex128 [batch][rec_capacity][3][n][n] or NULL: the FP64 RK4 stage
-                          operands X + h/2 k1, X + h/2 k2, X + h k3 of every recorded step
-                          (per-stage kernels only: set LIME_F_FORCE_TILED for n <= 96)     */
+  void* stage_states;  /* complex128 [rec_capacity][3][batch][n][n] (record-major) or NULL:
+                          the FP64 RK4 stage operands X + h/2 k1, X + h/2 k2, X + h k3 of
+                          every recorded step (per-stage kernels only: set
+                          LIME_F_FORCE_TILED for n <= 96)                                  */
 } lime_lindblad_batch;
 
 typedef struct lime_step_params {

@@ -109,13 +109,13 @@ class ChainQRT:
         """Recompute steps c0 .. c0+length-1 from the checkpoint at step c0.
 
         Returns X (B, length, N, N) with X[:, j] = X_{c0+j} and the FP64 stage operands
-        (B, length, 3, N, N) of the step leaving X_{c0+j}: X + h/2 k1, X + h/2 k2, X + h k3
+        (length, 3, B, N, N) of the step leaving X_{c0+j}: X + h/2 k1, X + h/2 k2, X + h k3
         -- the values the forward step used, recorded by the per-stage kernels."""
         B, n = self.B, self.n
         Xc = self.x0.clone() if c0 == 0 else self.checkpoints[:, c0 // self.K - 1].clone().contiguous()
         out = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
         out[:, 0] = Xc
-        stages = torch.empty((B, length, 3, n, n), dtype=torch.complex128, device=self.dev)
+        stages = torch.empty((length, 3, B, n, n), dtype=torch.complex128, device=self.dev)
         st = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
         self.rec_prop.step(Xc, self.dt, length, states=st, stage_states=stages, nout=1,
                            step_offset=0, rec_capacity=length)
@@ -150,7 +150,7 @@ class ChainQRT:
                 # lambda' = lambda_{k+1}: add the source of record k+1 (C_{k+1} = Tr[mu X_{k+1}])
                 lam.add_(G[:, k].reshape(B, 1, 1) * mu_h)
                 X = Xs[:, k - c0].contiguous()
-                s2, s3, s4 = (stages[:, k - c0, i].contiguous() for i in range(3))
+                s2, s3, s4 = stages[k - c0]                      # contiguous views
                 k4 = (h / 6.0) * lam
                 k4l = k4.to(opd).contiguous()
                 acc = lam.clone()

@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
           const double sx = r.x + cnext * (double)kr, sy = r.y + cnext * (double)ki;
           Sn[o] = CT<R>::make(sx, sy);
           if (a.stage_rec && a.rec >= 0)        // FP64 stage operand of this recorded step
-            a.stage_rec[(((size_t)b * a.rec_capacity + a.rec) * 3 + st) * nn + o] = make_double2(sx, sy);
+            a.stage_rec[(((size_t)a.rec * 3 + st) * a.batch + b) * nn + o] = make_double2(sx, sy);
         } else {
           r.x += (double)ic.x;
           r.y += (double)ic.y;

@@ -25,7 +25,7 @@ struct TiledArgs {
   void* inc;
   void* tbuf;
   double* tscale;          // per-operator 1 / Re Tr rho of the trace renormalisation
-  double2* stage_rec;      // [batch][rec_capacity][3][n][n] FP64 stage operands, or NULL
+  double2* stage_rec;      // [rec_capacity][3][batch][n][n] FP64 stage operands, or NULL
   // generator application (lindblad_apply_tiled): V = L(s_cur);
   //   y = c_lx V + c_z z (complex128), y64 = copy of y in the operand precision,
   //   acc += c_acc V

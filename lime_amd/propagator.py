@@ -272,9 +272,9 @@ class LindbladPropagator:
              nout=1, step_offset=0, rec_capacity=0, renorm=False, stream=None, stage_states=None):
         """Advance ``rho`` (B, N, N) complex128 in place by ``nsteps`` RK4 steps (async).
 
-        ``stage_states`` (B, rec_capacity, 3, N, N) complex128 receives the FP64 stage
-        operands X + h/2 k1, X + h/2 k2, X + h k3 of every recorded step (per-stage
-        kernels: the propagator must be ``force_tiled`` for n <= 96)."""
+        ``stage_states`` (rec_capacity, 3, B, N, N) complex128 (record-major) receives the
+        FP64 stage operands X + h/2 k1, X + h/2 k2, X + h k3 of every recorded step
+        (per-stage kernels: the propagator must be ``force_tiled`` for n <= 96)."""
         n, B = self.n, self.batch
         _check(rho, "rho", (B, n, n), torch.complex128, self.dev)
         eop_group = int(eop_group or B)
@@ -293,7 +293,7 @@ class LindbladPropagator:
         if states is not None:
             _check(states, "states", (B, int(rec_capacity), n, n), torch.complex128, self.dev)
         if stage_states is not None:
-            _check(stage_states, "stage_states", (B, int(rec_capacity), 3, n, n), torch.complex128, self.dev)
+            _check(stage_states, "stage_states", (int(rec_capacity), 3, B, n, n), torch.complex128, self.dev)
         d = _lib.LindbladBatch()
         d.batch, d.n, d.group, d.n_dense = B, n, self.group, self.n_dense
         d.n_eops = n_e
