@@ -376,3 +376,25 @@ def test_host_checks_reject_buffers_the_kernel_would_overrun():
     # the well-formed call still runs
     prop.step(rho, 0.1, 4, eops_t=E, eop_group=1, obs=obs, nout=1, rec_capacity=4)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_tiny_systems_and_empty_calls(n):
+    """n = 1 and 2 (one MFMA tile, mostly padding), zero steps and an empty batch."""
+    rng = np.random.default_rng(50 + n)
+    H = rand_herm(rng, n) if n > 1 else np.array([[0.3]], dtype=complex)
+    c_ops = [np.sqrt(0.1) * np.diag(np.eye(n)[i]) for i in range(n)]
+    if n > 1:
+        c_ops.append(0.2 * rand_c(rng, n, n))
+    X0 = rand_c(rng, n, n)
+    prop = _prop()(H, c_ops)
+    rho, _, _ = prop.run(X0, 0.05, 40)
+    ref, _, _ = ol.propagate(H, X0, c_ops, 0.05, 40)
+    assert np.max(np.abs(rho[0].cpu().numpy() - ref)) < 2e-7 * max(1.0, np.max(np.abs(ref)))
+    # zero steps: the state is returned unchanged (and never mutated in the caller's copy)
+    r0, _, _ = prop.run(X0, 0.05, 0)
+    assert np.array_equal(r0[0].cpu().numpy(), X0)
+    for tiled in (False, True):
+        p2 = _prop()(H, c_ops, force_tiled=tiled)
+        r2, _, _ = p2.run(X0, 0.05, 40)
+        assert np.max(np.abs(r2[0].cpu().numpy() - ref)) < 2e-7 * max(1.0, np.max(np.abs(ref)))
