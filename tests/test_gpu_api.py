@@ -248,3 +248,33 @@ def test_resolvent_on_gpu():
     L = osu.liouvillian(H, c_ops)
     R = so.resolvent(0.3 + 0.1j, L)
     assert np.max(np.abs(R @ ((0.3 + 0.1j) * np.eye(16) - L) - np.eye(16))) < 1e-12
+
+
+def test_correlation_3p_1t_writes_lime_files(golden_dir, tmp_path, monkeypatch):
+    """lime/correlation.py:17-69 driven as examples/test_cavity.py drives it: cor.dat and
+    dm.dat in lime's text formats, matching lime's own output files (fixture)."""
+    import scipy.sparse as sp
+    from lime_amd.correlation import correlation_3p_1t
+    from lime_amd.oqs import liouvillian as lindblad
+    monkeypatch.chdir(tmp_path)
+    g = np.load(os.path.join(golden_dir, "cavity_qrt.npz"))
+    n, kappa, n_th = 10, 0.25, 2.0
+    a = sp.csr_matrix(np.diag(np.sqrt(np.arange(1, n)), 1).astype(np.complex128))
+    num = sp.csr_matrix(np.diag(np.arange(n)).astype(np.complex128))
+    H = sp.csr_matrix(np.diag(np.arange(n) * 1.0).astype(np.complex128))
+    c_ops = [np.sqrt(kappa * (1 + n_th)) * a, np.sqrt(kappa * n_th) * a.conj().T]
+    w = np.exp(-np.arange(n) / n_th)
+    rho0 = sp.csr_matrix(np.diag(w / w.sum()).astype(np.complex128))
+    assert correlation_3p_1t(H, rho0, [a.conj().T, num, a], c_ops, np.linspace(0, 10, 200), lindblad) is None
+    cor_lines = open("cor.dat").read().splitlines()
+    dm_lines = open("dm.dat").read().splitlines()
+    assert len(cor_lines) == 200 and len(dm_lines) == 200
+    t = np.array([float(l.split()[0]) for l in cor_lines])
+    cor = np.array([complex(l.split()[1]) for l in cor_lines])
+    dm = np.array([[complex(x) for x in l.split()[1:]] for l in dm_lines])
+    assert dm.shape == (200, n * n)
+    assert np.allclose(t, g["t_cor"], rtol=0, atol=1e-12)
+    assert np.max(np.abs(cor - g["cor"])) < 2e-6
+    assert np.max(np.abs(dm.reshape(200, n, n) - g["dm"])) < 5e-7
+    with pytest.raises(ValueError):
+        correlation_3p_1t(H, rho0, [a.conj().T, num, a], c_ops, np.linspace(0, 10, 200), np.sin)
