@@ -414,6 +414,155 @@ __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
       }
 }
 
+// ---- per-element epilogues shared by the MFMA tiles and the border workgroups
+
+// RK4 stage update of element o of operator b from C = (cre, cim) (k = -i C + D o S):
+// the FP32/FP64 increment, the next stage operand, and at the last stage rho += increment
+// and the records.  Returns the final rho of the element at the last stage (else 0).
+template <typename R>
+__device__ __forceinline__ double2 stage_elem(const TiledArgs& a, int b, int set, size_t o, R cre, R cim) {
+  using C = typename CT<R>::C;
+  const size_t nn = (size_t)a.n * a.n;
+  const int st = a.stage;
+  const double dt = a.dt;
+  const R cacc = (R)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
+  const double cnext = (st == 2) ? dt : 0.5 * dt;
+  const size_t ob = (size_t)b * nn + o;
+  R kr = cim, ki = -cre;
+  if (a.dmat) {
+    const C d = static_cast<const C*>(a.dmat)[(size_t)set * nn + o];
+    const C x = static_cast<const C*>(a.s_cur)[ob];
+    kr += d.x * x.x - d.y * x.y;
+    ki += d.x * x.y + d.y * x.x;
+  }
+  C* inc = static_cast<C*>(a.inc);
+  C* Sn = static_cast<C*>(a.s_next);
+  C ic;
+  if (st == 0) {
+    ic.x = 0;
+    ic.y = 0;
+  } else {
+    ic = inc[ob];
+  }
+  ic.x += cacc * kr;
+  ic.y += cacc * ki;
+  double2 r = a.rho[ob];
+  if (st < 3) {
+    inc[ob] = ic;
+    const double sx = r.x + cnext * (double)kr, sy = r.y + cnext * (double)ki;
+    Sn[ob] = CT<R>::make(sx, sy);
+    if (a.stage_rec && a.rec >= 0)        // FP64 stage operand of this recorded step
+      a.stage_rec[(((size_t)a.rec * 3 + st) * a.batch + b) * nn + o] = make_double2(sx, sy);
+    return make_double2(0.0, 0.0);
+  }
+  r.x += (double)ic.x;
+  r.y += (double)ic.y;
+  a.rho[ob] = r;
+  Sn[ob] = CT<R>::make(r.x, r.y);
+  if (a.rec >= 0 && a.states && !(a.flags & 1)) a.states[((size_t)b * a.rec_capacity + a.rec) * nn + o] = r;
+  return r;
+}
+
+// Generator application epilogue of element o: V = -i C + D o X; y = c_lx V + c_z z,
+// y_op = y in the operand precision, acc += c_acc V.
+template <typename R>
+__device__ __forceinline__ void apply_elem(const TiledArgs& a, int b, int set, size_t o, R cre, R cim) {
+  using C = typename CT<R>::C;
+  const size_t nn = (size_t)a.n * a.n;
+  const size_t ob = (size_t)b * nn + o;
+  double vr = (double)cim, vi = -(double)cre;
+  if (a.dmat) {
+    const C d = static_cast<const C*>(a.dmat)[(size_t)set * nn + o];
+    const C x = static_cast<const C*>(a.s_cur)[ob];
+    vr += (double)(d.x * x.x - d.y * x.y);
+    vi += (double)(d.x * x.y + d.y * x.x);
+  }
+  C* yop = static_cast<C*>(a.y64);
+  if (a.y || yop) {
+    double yr = a.c_lx.x * vr - a.c_lx.y * vi;
+    double yi = a.c_lx.x * vi + a.c_lx.y * vr;
+    if (a.z) {
+      const double2 zz = a.z[ob];
+      yr += a.c_z.x * zz.x - a.c_z.y * zz.y;
+      yi += a.c_z.x * zz.y + a.c_z.y * zz.x;
+    }
+    if (a.y) a.y[ob] = make_double2(yr, yi);
+    if (yop) yop[ob] = CT<R>::make(yr, yi);
+  }
+  if (a.acc) {
+    double2 t = a.acc[ob];
+    t.x += a.c_acc.x * vr - a.c_acc.y * vi;
+    t.y += a.c_acc.x * vi + a.c_acc.y * vr;
+    a.acc[ob] = t;
+  }
+}
+
+// Adds the workgroup's share (pr, pi: this lane's part) of Tr[E_e rho] at the record of
+// this step to the observables with FP64 atomics.  Called by every thread of the group.
+__device__ __forceinline__ void obs_atomic(const TiledArgs& a, int b, double pr, double pi, int e,
+                                           double2* sRed) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double sr = wave_sum(pr), si = wave_sum(pi);
+  __syncthreads();
+  if (lane == 0) sRed[w] = make_double2(sr, si);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t = make_double2(0.0, 0.0);
+    for (int x = 0; x < 4; ++x) {
+      t.x += sRed[x].x;
+      t.y += sRed[x].y;
+    }
+    double* dst = reinterpret_cast<double*>(a.obs + ((size_t)b * a.rec_capacity + a.rec) * a.n_eops + e);
+    atomicAdd(dst, t.x);
+    atomicAdd(dst + 1, t.y);
+  }
+}
+
+// ---- bordered operators (n = nc + 1, nc a multiple of TILE, no dense operators): the
+// MFMA tiles cover the nc x nc core -- their reduction runs over all n indices, so the
+// border terms of core outputs are included -- and BORDER_WGS extra workgroups per
+// operator form the 2 nc + 1 outputs of row nc and column nc as complex dot products,
+// BG lanes per output (shuffle-reduced), BG_OUT outputs per workgroup.
+constexpr int BG = 8;
+constexpr int BG_OUT = 256 / BG;
+
+// C = (H S - S H^dag)_(bi, bj) over the lanes q, q + BG, ... of this output's group
+template <typename R>
+__device__ __forceinline__ void border_dot(const TiledArgs& a, int b, int set, int bi, int bj, int q, bool own,
+                                           R& cre, R& cim) {
+  using C = typename CT<R>::C;
+  const int n = a.n;
+  const size_t nn = (size_t)n * n;
+  const C* H = static_cast<const C*>(a.heff) + (size_t)set * nn;
+  const C* S = static_cast<const C*>(a.s_cur) + (size_t)b * nn;
+  R xr = 0, xi = 0;
+  if (own) {
+    for (int k = q; k < n; k += BG) {
+      const C h = H[(size_t)bi * n + k], sv = S[(size_t)k * n + bj];
+      const C z = S[(size_t)bi * n + k], g = H[(size_t)bj * n + k];
+      xr += h.x * sv.x - h.y * sv.y - (z.x * g.x + z.y * g.y);
+      xi += h.x * sv.y + h.y * sv.x - (z.y * g.x - z.x * g.y);
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < BG; off <<= 1) {
+    xr += __shfl_xor(xr, off, 64);
+    xi += __shfl_xor(xi, off, 64);
+  }
+  cre = xr;
+  cim = xi;
+}
+
+__device__ __forceinline__ void border_index(int nc, int oidx, int& bi, int& bj) {
+  if (oidx <= nc) {
+    bi = nc;
+    bj = oidx;
+  } else {
+    bi = oidx - nc - 1;
+    bj = nc;
+  }
+}
+
 // One RK4 stage for every operator of the batch.
 template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
@@ -421,15 +570,43 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
   using V4 = typename CT<R>::V4;
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   __shared__ double2 sRed[4];
-  const int n = a.n;
-  const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int n = a.n, nc = a.nc;
+  const int tiles = tiles_per_dim(nc, TILE) * tiles_per_dim(nc, TILE);
+  const int per = tiles + a.border_wgs;
   const int blk = xcd_block();
-  const int b = blk / per;
-  const Geo g = geometry<TILE>(n, blk % per);
+  const int b = blk / per, idx = blk % per;
   const int set = b / a.group;
   const size_t nn = (size_t)n * n;
-  constexpr int BW = TILE / 32;
+  const bool rec = (a.stage == 3) && a.rec >= 0;
 
+  if (idx >= tiles) {                         // border workgroup (uniform branch)
+    const int q = threadIdx.x % BG;
+    const int oidx = (idx - tiles) * BG_OUT + threadIdx.x / BG;
+    const bool own = oidx < 2 * nc + 1;
+    int bi = 0, bj = 0;
+    if (own) border_index(nc, oidx, bi, bj);
+    R cre, cim;
+    border_dot<R>(a, b, set, bi, bj, q, own, cre, cim);
+    double2 rf = make_double2(0.0, 0.0);
+    const bool lead = own && q == 0;
+    if (lead) rf = stage_elem<R>(a, b, set, (size_t)bi * n + bj, cre, cim);
+    if (rec && a.obs) {
+      const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
+      for (int e = 0; e < a.n_eops; ++e) {
+        double pr = 0.0, pi = 0.0;
+        if (lead) {
+          const double2 ev = E0[(size_t)e * nn + (size_t)bi * n + bj];
+          pr = ev.x * rf.x - ev.y * rf.y;
+          pi = ev.x * rf.y + ev.y * rf.x;
+        }
+        obs_atomic(a, b, pr, pi, e, sRed);
+      }
+    }
+    return;
+  }
+
+  const Geo g = geometry<TILE>(nc, idx);
+  constexpr int BW = TILE / 32;
   V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
   for (int p = 0; p < BW; ++p)
@@ -438,16 +615,6 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
   generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci);
 
   // ---- RK4 stage epilogue
-  const int st = a.stage;
-  const double dt = a.dt;
-  const R cacc = (R)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
-  const double cnext = (st == 2) ? dt : 0.5 * dt;
-  const C* S = static_cast<const C*>(a.s_cur) + (size_t)b * nn;
-  double2* rho = a.rho + (size_t)b * nn;
-  C* inc = static_cast<C*>(a.inc) + (size_t)b * nn;
-  C* Sn = static_cast<C*>(a.s_next) + (size_t)b * nn;
-  const C* Dm = a.dmat ? static_cast<const C*>(a.dmat) + (size_t)set * nn : nullptr;
-  const bool rec = (st == 3) && a.rec >= 0;
   double2 rfin[BW][BW][4];
 #pragma unroll
   for (int p = 0; p < BW; ++p)
@@ -458,38 +625,8 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
         rfin[p][q][v] = make_double2(0.0, 0.0);
         const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
         const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-        if (i >= n || j >= n) continue;
-        const size_t o = (size_t)i * n + j;
-        R kr = ci[p][q][v], ki = -cr[p][q][v];
-        if (Dm) {
-          const C d = Dm[o], x = S[o];
-          kr += d.x * x.x - d.y * x.y;
-          ki += d.x * x.y + d.y * x.x;
-        }
-        C ic;
-        if (st == 0) {
-          ic.x = 0;
-          ic.y = 0;
-        } else {
-          ic = inc[o];
-        }
-        ic.x += cacc * kr;
-        ic.y += cacc * ki;
-        double2 r = rho[o];
-        if (st < 3) {
-          inc[o] = ic;
-          const double sx = r.x + cnext * (double)kr, sy = r.y + cnext * (double)ki;
-          Sn[o] = CT<R>::make(sx, sy);
-          if (a.stage_rec && a.rec >= 0)        // FP64 stage operand of this recorded step
-            a.stage_rec[(((size_t)a.rec * 3 + st) * a.batch + b) * nn + o] = make_double2(sx, sy);
-        } else {
-          r.x += (double)ic.x;
-          r.y += (double)ic.y;
-          rho[o] = r;
-          rfin[p][q][v] = r;
-          Sn[o] = CT<R>::make(r.x, r.y);
-          if (rec && a.states && !(a.flags & 1)) a.states[((size_t)b * a.rec_capacity + a.rec) * nn + o] = r;
-        }
+        if (i >= nc || j >= nc) continue;
+        rfin[p][q][v] = stage_elem<R>(a, b, set, (size_t)i * n + j, cr[p][q][v], ci[p][q][v]);
       }
   if (rec && a.obs) {
     const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
@@ -503,25 +640,12 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
           for (int v = 0; v < 4; ++v) {
             const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
             const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-            if (i >= n || j >= n) continue;
+            if (i >= nc || j >= nc) continue;
             const double2 ev = E0[(size_t)e * nn + (size_t)i * n + j];
             pr += ev.x * rfin[p][q][v].x - ev.y * rfin[p][q][v].y;
             pi += ev.x * rfin[p][q][v].y + ev.y * rfin[p][q][v].x;
           }
-      const double sr = wave_sum(pr), si = wave_sum(pi);
-      __syncthreads();
-      if (g.lane == 0) sRed[g.w] = make_double2(sr, si);
-      __syncthreads();
-      if (g.tid == 0) {
-        double2 t = make_double2(0.0, 0.0);
-        for (int x = 0; x < 4; ++x) {
-          t.x += sRed[x].x;
-          t.y += sRed[x].y;
-        }
-        double* dst = reinterpret_cast<double*>(a.obs + ((size_t)b * a.rec_capacity + a.rec) * a.n_eops + e);
-        atomicAdd(dst, t.x);
-        atomicAdd(dst + 1, t.y);
-      }
+      obs_atomic(a, b, pr, pi, e, sRed);
     }
   }
 }
@@ -531,16 +655,28 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
 // acc += c_acc V.
 template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
-  using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
-  const int n = a.n;
-  const int per = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int n = a.n, nc = a.nc;
+  const int tiles = tiles_per_dim(nc, TILE) * tiles_per_dim(nc, TILE);
+  const int per = tiles + a.border_wgs;
   const int blk = xcd_block();
-  const int b = blk / per;
-  const Geo g = geometry<TILE>(n, blk % per);
+  const int b = blk / per, idx = blk % per;
   const int set = b / a.group;
-  const size_t nn = (size_t)n * n;
+
+  if (idx >= tiles) {                         // border workgroup (uniform branch)
+    const int q = threadIdx.x % BG;
+    const int oidx = (idx - tiles) * BG_OUT + threadIdx.x / BG;
+    const bool own = oidx < 2 * nc + 1;
+    int bi = 0, bj = 0;
+    if (own) border_index(nc, oidx, bi, bj);
+    R cre, cim;
+    border_dot<R>(a, b, set, bi, bj, q, own, cre, cim);
+    if (own && q == 0) apply_elem<R>(a, b, set, (size_t)bi * n + bj, cre, cim);
+    return;
+  }
+
+  const Geo g = geometry<TILE>(nc, idx);
   constexpr int BW = TILE / 32;
   V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
@@ -548,9 +684,6 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
   generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci);
-  const C* Dm = a.dmat ? static_cast<const C*>(a.dmat) + (size_t)set * nn : nullptr;
-  const C* X = static_cast<const C*>(a.s_cur);
-  C* yop = static_cast<C*>(a.y64);
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
@@ -559,31 +692,8 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
       for (int v = 0; v < 4; ++v) {
         const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
         const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-        if (i >= n || j >= n) continue;
-        const size_t o = (size_t)b * nn + (size_t)i * n + j;
-        double vr = (double)ci[p][q][v], vi = -(double)cr[p][q][v];
-        if (Dm) {
-          const C d = Dm[(size_t)i * n + j], x = X[o];
-          vr += (double)(d.x * x.x - d.y * x.y);
-          vi += (double)(d.x * x.y + d.y * x.x);
-        }
-        if (a.y || yop) {
-          double yr = a.c_lx.x * vr - a.c_lx.y * vi;
-          double yi = a.c_lx.x * vi + a.c_lx.y * vr;
-          if (a.z) {
-            const double2 zz = a.z[o];
-            yr += a.c_z.x * zz.x - a.c_z.y * zz.y;
-            yi += a.c_z.x * zz.y + a.c_z.y * zz.x;
-          }
-          if (a.y) a.y[o] = make_double2(yr, yi);
-          if (yop) yop[o] = CT<R>::make(yr, yi);
-        }
-        if (a.acc) {
-          double2 t = a.acc[o];
-          t.x += a.c_acc.x * vr - a.c_acc.y * vi;
-          t.y += a.c_acc.x * vi + a.c_acc.y * vr;
-          a.acc[o] = t;
-        }
+        if (i >= nc || j >= nc) continue;
+        apply_elem<R>(a, b, set, (size_t)i * n + j, cr[p][q][v], ci[p][q][v]);
       }
 }
 
@@ -690,10 +800,20 @@ hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Bordered layout when n = nc + 1 with nc a multiple of the tile and no dense operators
+// (their products T_k = L_k S would need a border of their own); sets a.nc, a.border_wgs.
+// Returns the workgroups per operator of the stage / apply kernels.
+static int layout(TiledArgs& a, int tile) {
+  const bool brd = a.n_dense == 0 && a.n > tile && (a.n - 1) % tile == 0;
+  a.nc = brd ? a.n - 1 : a.n;
+  a.border_wgs = brd ? (2 * a.nc + 1 + BG_OUT - 1) / BG_OUT : 0;
+  const int tpd = (a.nc + tile - 1) / tile;
+  return tpd * tpd + a.border_wgs;
+}
+
 template <int TILE, typename R>
 static hipError_t apply_impl(TiledArgs a, hipStream_t stream) {
-  const int tpd = (a.n + TILE - 1) / TILE;
-  const int per = tpd * tpd;
+  const int per = layout(a, TILE);
   if (a.n_dense > 0)
     hipLaunchKernelGGL((lindblad_left_tiled<TILE, R, 0>), dim3(per * a.batch * a.n_dense), dim3(256), 0, stream, a);
   hipLaunchKernelGGL((lindblad_apply_tiled<TILE, R, 0>), dim3(per * a.batch), dim3(256), 0, stream, a);
@@ -707,8 +827,7 @@ hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64) {
 
 template <int TILE, typename R, int P>
 static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) {
-  const int tpd = (a.n + TILE - 1) / TILE;
-  const int per = tpd * tpd;
+  const int per = layout(a, TILE);
   for (int st = 0; st < 4; ++st) {
     a.stage = st;
     a.s_cur = (st % 2 == 0) ? s0 : s1;

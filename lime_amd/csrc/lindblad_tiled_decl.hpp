@@ -8,6 +8,7 @@ namespace lime {
 
 struct TiledArgs {
   int batch, n, group, n_dense, n_eops, eop_group, flags;
+  int nc, border_wgs;      // tile-covered core size and border workgroups per operator
   int stage, rec, rec_capacity;
   double dt;
   // operator sets and workspace are complex64 (FP32 variant) or complex128 (FP64 variant)

@@ -65,7 +65,8 @@ def problem(ns, B, seed):
 # for the per-site gradients (eps, J) and reaches ~1e-5 on the shared-gamma gradient, a
 # sum over every site of FP32-accumulated terms (gated at 5e-5).
 @pytest.mark.parametrize("ns,nt,K,prec", [(5, 120, 16, "fp32"), (12, 64, 64, "fp32"), (100, 24, 8, "fp32"),
-                                          (12, 64, 16, "fp64"), (100, 24, 8, "fp64")])
+                                          (12, 64, 16, "fp64"), (100, 24, 8, "fp64"),
+                                          (128, 12, 4, "fp32")])     # n = 129: bordered tiles
 def test_adjoint_gradient_matches_fp64_reference_and_fd(ns, nt, K, prec):
     from lime_amd.adjoint import ChainAbsorption
     B, dt = 2, 0.05

@@ -398,3 +398,45 @@ def test_tiny_systems_and_empty_calls(n):
         p2 = _prop()(H, c_ops, force_tiled=tiled)
         r2, _, _ = p2.run(X0, 0.05, 40)
         assert np.max(np.abs(r2[0].cpu().numpy() - ref)) < 2e-7 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("n", [33, 65, 97, 129])
+@pytest.mark.parametrize("prec", ["fp32", "fp64", "3xbf16"])
+def test_bordered_tiled_layout(n, prec):
+    """Per-stage kernels at n = nc + 1 with nc a multiple of the tile: the MFMA tiles cover
+    the nc x nc core, border workgroups the last row and column (both tile sizes; with
+    observables, state records and stage-operand records)."""
+    rng = np.random.default_rng(800 + n)
+    H = rand_herm(rng, n)
+    c_ops = [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_dm(rng, n), rand_c(rng, n, n) / n])
+    A = [rand_herm(rng, n)]
+    dt, nt = 0.02, 6
+    refs = [ol.propagate(H, X0[b], c_ops, dt, nt, A, nout=2, store_states=True) for b in range(2)]
+    tol = {"fp32": 2e-7, "fp64": 1e-13, "3xbf16": 2e-6}[prec]
+    for v in (1, 2):                              # 32 x 32 and 64 x 64 tiles
+        prop = _prop()(H, c_ops, group=2, precision=prec, force_tiled=True, variant=v)
+        rho, obs, states = prop.run(X0, dt, nt, e_ops=A, nout=2, store_states=True)
+        for b in range(2):
+            ref, ref_obs, ref_states = refs[b]
+            scale = np.max(np.abs(ref))
+            err = np.max(np.abs(rho[b].cpu().numpy() - ref)) / scale
+            print(f"bordered tiled n={n} {prec} tile{32 * v} b={b} rel err {err:.3e}")
+            assert err < tol
+            assert np.max(np.abs(obs[b].cpu().numpy() - ref_obs)) < 10 * tol * np.max(np.abs(ref_obs))
+            assert np.max(np.abs(states[b].cpu().numpy() - np.array(ref_states))) < tol * scale
+    # stage-operand records: the FP64 stage operands X + h/2 k1, X + h/2 k2, X + h k3
+    if prec == "fp64":
+        prop = _prop()(H, c_ops, group=2, precision=prec, force_tiled=True)
+        Xt = torch.as_tensor(X0, device="cuda").contiguous()
+        stg = torch.empty((1, 3, 2, n, n), dtype=torch.complex128, device="cuda")
+        prop.step(Xt, dt, 1, stage_states=stg, nout=1, rec_capacity=1)
+        for b in range(2):
+            f = lambda x: ol.rhs_heff(x, *ol.split_ops(H, c_ops))
+            k1 = f(X0[b])
+            s2 = X0[b] + 0.5 * dt * k1
+            k2 = f(s2)
+            s3 = X0[b] + 0.5 * dt * k2
+            s4 = X0[b] + dt * f(s3)
+            for i, ref in enumerate((s2, s3, s4)):
+                assert np.max(np.abs(stg[0, i, b].cpu().numpy() - ref)) < 1e-13
