@@ -8,10 +8,15 @@
 * ``propagate``       -- ``lime/oqs.py:373-468`` (``_redfield``): RK4 on row-major vec rho
   in the eigenbasis; states are transformed back with ``v rho v^dag``; observables are
   taken in the eigenbasis (transformed e_ops).
+* ``propagator_expm`` -- ``lime/oqs.py:169-209``: U(t) = exp(R t) stacked as (N^2, N^2, Nt).
+* ``response_4op_3t`` -- ``lime/oqs.py:277-366``: <<I| a G(t_i) b G(t_j) c G(t_k) d |rho0>>,
+  G = -i U.
 
 Pinned by: the Liouville form against the Hilbert form (independent constructions);
 the population block of the secular Redfield rate matrix against golden-rule rates
-2|A_nm|^2 S(w) (closed form); trace conservation.
+2|A_nm|^2 S(w) (closed form); trace conservation; exp(R t) against the eigendecomposition
+V e^{Lambda t} V^-1 and the semigroup law; the four-point response against the closed form
+of a single two-level Liouville pathway with distinct rates (time ordering, the -i of G).
 """
 import numpy as np
 from scipy.linalg import eigh

@@ -3,10 +3,11 @@
 * ``fft``  -- ``lime/fft.py:9-56``: g(w) = sum_n f(t_n) e^{-i w t_n} dx, fftshifted,
               w = 2 pi fftshift(fftfreq(n, dx)), times the phase e^{-i w x0}.
 * ``ifft`` -- ``lime/fft.py:58-78``: g = (1/2pi) sum f e^{+i w t} dx.
-* ``fft2`` -- ``lime/fft.py:80-102``.
+* ``fft2`` -- ``lime/fft.py:80-102`` (the y frequencies come from the x length, as there).
 
 Pinned by: the analytic transform of a damped exponential (a Lorentzian) sampled on the
-same grid -- a closed form independent of numpy's FFT.
+same grid -- a closed form independent of numpy's FFT; for ``fft2`` the product of two such
+closed forms for a separable exponential.
 """
 import numpy as np
 

@@ -334,3 +334,69 @@ def test_driven_samples_field_at_step_end():
     r1, _, _ = ol.propagate_driven([np.diag(E), [np.diag(d), lambda t: 0.3]], rho0, c_ops, dt, 20)
     r2, _, _ = ol.propagate(np.diag(E) - 0.3 * np.diag(d), rho0, c_ops, dt, 20)
     assert np.max(np.abs(r1 - r2)) < 1e-13
+
+
+def test_propagator_expm_eigen_and_semigroup():
+    """exp(R t) against V e^{Lambda t} V^-1 of a random non-normal R, U(0) = 1 and the
+    semigroup law U(t1 + t2) = U(t1) U(t2)."""
+    rng = np.random.default_rng(14)
+    m = 9
+    R = 0.3 * rand_c(rng, m, m)
+    t = np.array([0.0, 0.4, 1.1, 1.5])
+    U = orf.propagator_expm(R, t)
+    assert U.shape == (m, m, 4)
+    lam, V = np.linalg.eig(R)
+    Vi = np.linalg.inv(V)
+    for k, tk in enumerate(t):
+        expect = (V * np.exp(lam * tk)) @ Vi
+        assert np.max(np.abs(U[:, :, k] - expect)) < 1e-11
+    assert np.max(np.abs(U[:, :, 0] - np.eye(m))) < 1e-15
+    assert np.max(np.abs(U[:, :, 3] - U[:, :, 1] @ U[:, :, 2])) < 1e-11
+
+
+def test_response_4op_3t_time_ordering_closed_form():
+    """Two levels, R diagonal in the row-major Liouville basis with distinct rates, all four
+    superoperators sigma_x from the left or right.  The pathway |0><0| -d-> |1><0| (t_k)
+    -c-> |1><1| (t_j) -b-> |0><1| (t_i) -a-> |1><1| gives (-i)^3 exp(l01 t_i + l11 t_j + l10 t_k):
+    pins which time goes with which propagator, the -i of G and the left/right (kron)
+    convention."""
+    sx = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    eye = np.eye(2)
+    left = np.kron(sx, eye)           # vec(A X) for row-major vec
+    right = np.kron(eye, sx.T)        # vec(X A)
+    lam = np.array([-0.05, -0.3 - 1.1j, -0.2 + 0.7j, -0.45])   # l00, l01, l10, l11
+    R = np.diag(lam)
+    rho0 = np.diag([1.0, 0.0]).astype(np.complex128)
+    t = np.array([0.0, 0.3, 0.8, 1.7, 2.2])
+    out = orf.response_4op_3t(R, rho0, [left, left, right, left], t)
+    ti, tj, tk = np.meshgrid(t, t, t, indexing="ij")
+    expect = (-1j) ** 3 * np.exp(lam[1] * ti + lam[3] * tj + lam[2] * tk)
+    assert np.max(np.abs(out - expect)) < 1e-13
+    # R = 0: G = -i for every time, and the response is (-i)^3 Tr[unvec(a b c d vec rho0)]
+    rng = np.random.default_rng(15)
+    ops = [rand_c(rng, 9, 9) for _ in range(4)]
+    rho_r = rand_dm(rng, 3)
+    r0 = orf.response_4op_3t(np.zeros((9, 9)), rho_r, ops, np.array([0.0, 1.0]))
+    v = ops[0] @ ops[1] @ ops[2] @ ops[3] @ rho_r.reshape(-1)
+    assert np.max(np.abs(r0 - (-1j) ** 3 * np.trace(v.reshape(3, 3)))) < 1e-12
+
+
+def test_fft2_separable_geometric_series():
+    """A separable damped exponential on an nx x nx grid transforms to the product of two
+    geometric-series closed forms; lime's fft2 takes the y frequencies from nx (lime/fft.py:100),
+    which a rectangular grid exposes."""
+    n, dx, dy = 64, 0.1, 0.25
+    a, b, wa, wb = 0.3, 0.15, 1.2, -0.7
+    x = dx * np.arange(n)
+    y = dy * np.arange(n)
+    f = np.outer(np.exp(-(a + 1j * wa) * x), np.exp(-(b + 1j * wb) * y))
+    fx, fy, G = osp.fft2(f, dx, dy)
+    kk = np.arange(-n // 2, n // 2)
+    assert np.allclose(fx, 2 * np.pi * kk / (n * dx), atol=1e-12)
+    assert np.allclose(fy, 2 * np.pi * kk / (n * dy), atol=1e-12)
+    zx = np.exp(-(a + 1j * (wa + fx)) * dx)
+    zy = np.exp(-(b + 1j * (wb + fy)) * dy)
+    expect = np.outer(dx * (1 - zx ** n) / (1 - zx), dy * (1 - zy ** n) / (1 - zy))
+    assert np.max(np.abs(G - expect)) < 1e-11
+    fx2, fy2, _ = osp.fft2(np.ones((8, 4)), 1.0, 1.0)
+    assert len(fx2) == 8 and len(fy2) == 8
