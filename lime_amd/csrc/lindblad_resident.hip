@@ -27,6 +27,11 @@ namespace lime {
 
 namespace {
 
+// D o S at one element (complex product d s)
+__device__ __forceinline__ float2 cmulf(float2 d, float2 s) {
+  return make_float2(d.x * s.x - d.y * s.y, d.x * s.y + d.y * s.x);
+}
+
 // Copy a row-major n x n complex64 operator into an NP x P LDS buffer (zero padded).
 template <int NP, int P, int T>
 __device__ __forceinline__ void stage_operator(float2* dst, const float2* __restrict__ src, int n, int tid) {
@@ -172,7 +177,7 @@ lindblad_rk4_resident(ResidentArgs a) {
 #pragma unroll
   for (int u = 0; u < BPW; ++u) {
     const int t = w + u * W;
-    tval[u] = t < NT;
+    tval[u] = (W * BPW == NT) || t < NT;   // every slot owns a tile when W * BPW == NT
     const int tt = tval[u] ? t : NT - 1;   // idle slots shadow a real tile, never store
     trow[u] = (tt / TG) * 16;
     tcol[u] = (tt % TG) * 16;
@@ -251,6 +256,31 @@ lindblad_rk4_resident(ResidentArgs a) {
   }
   __syncthreads();
 
+  // C accumulators of the stage.  DFOLD (D cached in registers): D o S is their initial
+  // value, formed by the epilogue that wrote S -- every lane writes the S elements it owns,
+  // so no stage reads S back for the Hadamard term.  br, bm: C at the border output
+  // (D o S only on the group's lane 0, since the group's partial sums are added up).
+  constexpr bool DFOLD = CACHE;
+  floatx4 cr[BPW], ci[BPW];
+  float br = 0.f, bm = 0.f;
+  // the accumulators hold C and the stage derivative is k = -i C, so D o S enters C as
+  // (Re C, Im C) = (-Im(D o S), Re(D o S))
+  if constexpr (DFOLD) {
+#pragma unroll
+    for (int u = 0; u < BPW; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float2 p = cmulf(dc[u][v], make_float2((float)rho[u][v].x, (float)rho[u][v].y));
+        cr[u][v] = -p.y;
+        ci[u][v] = p.x;
+      }
+    if (BRD && bwrite) {
+      const float2 p = cmulf(dc_b, make_float2((float)rho_b.x, (float)rho_b.y));
+      br = -p.y;
+      bm = p.x;
+    }
+  }
+
   const double dt = a.dt;
   const bool renorm = (a.flags & 1) != 0;
   const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
@@ -265,15 +295,20 @@ lindblad_rk4_resident(ResidentArgs a) {
       // rebuilt inside the loop instead of being hoisted (and spilled) as 64-bit values.
       int olr = lr, olg = lg;
       asm volatile("" : "+v"(olr), "+v"(olg));
-      floatx4 cr[BPW], ci[BPW];
+      if constexpr (!DFOLD) {
 #pragma unroll
-      for (int u = 0; u < BPW; ++u) {
-        cr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-        ci[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < BPW; ++u) {
+          cr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+          ci[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        br = 0.f;
+        bm = 0.f;
       }
       // C += H_eff S  and  C -= S H_eff^dag  ((S H^dag)_ij = sum_k S_ik conj(H_jk)).
       // Software pipelined by hand: fragments of step k0+4 are read from LDS while the
-      // MFMAs of step k0 issue, so at most two fragment sets are live.
+      // MFMAs of step k0 issue, so at most two fragment sets are live.  Bordered: the
+      // border dot products take their terms k0..k0+3 in the same step, on VALU while the
+      // MFMAs run (lane bq of an output's group takes k = k0 + bq + BG j).
       {
         int oR[BPW], oC[BPW];
 #pragma unroll
@@ -334,6 +369,20 @@ lindblad_rk4_resident(ResidentArgs a) {
               ci[u] = mfma_f32(-z[u].y, h[u].x, ci[u]);
             }
           }
+          if constexpr (BRD) {
+            if (bown) {
+#pragma unroll
+              for (int j = 0; j < 4 / BG; ++j) {
+                const int k = k0 + bq + BG * j;
+                if (k <= NC) {
+                  const float2 hh = sH[bi * P + k], ss = sS[k * P + bj];
+                  const float2 zz = sS[bi * P + k], gg = sH[bj * P + k];
+                  br += hh.x * ss.x - hh.y * ss.y - (zz.x * gg.x + zz.y * gg.y);
+                  bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
+                }
+              }
+            }
+          }
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
             x[u] = xn[u];
@@ -350,22 +399,11 @@ lindblad_rk4_resident(ResidentArgs a) {
           }
         }
       }
-      // bordered: C at the border output, (H S - S H^dag)_ij over k = 0..NC
-      float br = 0.f, bm = 0.f;
+      // bordered: C at the border output, (H S - S H^dag)_ij over k = 0..NC, summed over
+      // the output's lane group
       if (BRD) {
-        if (bown) {
-          for (int k = bq; k <= NC; k += BG) {
-            const float2 hh = sH[bi * P + k], ss = sS[k * P + bj];
-            const float2 zz = sS[bi * P + k], gg = sH[bj * P + k];
-            br += hh.x * ss.x - hh.y * ss.y - (zz.x * gg.x + zz.y * gg.y);
-            bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
-          }
-        }
-#pragma unroll
-        for (int off = 1; off < BG; off <<= 1) {
-          br += __shfl_xor(br, off, 64);
-          bm += __shfl_xor(bm, off, 64);
-        }
+        br = group_sum<BG>(br);
+        bm = group_sum<BG>(bm);
       }
       // C += i sum_k (L_k S) R_k^dag, every factor staged through LDS (sL) so the fragment
       // loops are the same pipelined LDS reads as the H_eff products.
@@ -396,7 +434,7 @@ lindblad_rk4_resident(ResidentArgs a) {
         lds_gemm<NP, P, BPW, 2>(sT, sL, trow, tcol, lr, lg, ke, cr, ci);    // C += i T R^dag
       }
       // k = -i C + D o S, formed in place: afterwards kr = ci, ki = -cr.
-      if (has_d) {
+      if (!DFOLD && has_d) {
         const float2* Dm = a.dmat + (size_t)set * nn;
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
@@ -427,46 +465,55 @@ lindblad_rk4_resident(ResidentArgs a) {
       if (!DB) __syncthreads();  // every wave has finished reading S
 
       const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
-      const double cnext = (st == 2) ? dt : 0.5 * dt;
+      const float keep = (st == 0) ? 0.f : 1.f;   // the increment restarts at stage 0
+      if (st < 3) {
+        // next stage operand S = rho + c k (FP64, rounded to complex64 into LDS)
+        const double cnext = (st == 2) ? dt : 0.5 * dt;
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float kr = ci[u][v], ki = -cr[u][v];
+            inc[u][v].x = fmaf(cacc, kr, keep * inc[u][v].x);
+            inc[u][v].y = fmaf(cacc, ki, keep * inc[u][v].y);
+            const float2 sv = make_float2((float)(rho[u][v].x + cnext * (double)kr),
+                                          (float)(rho[u][v].y + cnext * (double)ki));
+            if (tval[u]) sN[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = sv;
+            if constexpr (DFOLD) {
+              const float2 p = tval[u] ? cmulf(dc[u][v], sv) : make_float2(0.f, 0.f);
+              cr[u][v] = -p.y;
+              ci[u][v] = p.x;
+            }
+          }
+        }
+        if (BRD) {
+          const float kr = bm, ki = -br;
+          inc_b.x = fmaf(cacc, kr, keep * inc_b.x);
+          inc_b.y = fmaf(cacc, ki, keep * inc_b.y);
+          const float2 sv = make_float2((float)(rho_b.x + cnext * (double)kr), (float)(rho_b.y + cnext * (double)ki));
+          if (bwrite) sN[bi * P + bj] = sv;
+          if constexpr (DFOLD) {
+            const float2 p = bwrite ? cmulf(dc_b, sv) : make_float2(0.f, 0.f);
+            br = -p.y;
+            bm = p.x;
+          }
+        }
+        __syncthreads();  // S of the next stage is complete
+        ph ^= (int)DB;
+        continue;
+      }
+      // last stage: the state update itself is FP64
 #pragma unroll
       for (int u = 0; u < BPW; ++u) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const float kr = ci[u][v], ki = -cr[u][v];
-          float2 ic = (st == 0) ? make_float2(0.f, 0.f) : inc[u][v];
-          ic.x = fmaf(cacc, kr, ic.x);
-          ic.y = fmaf(cacc, ki, ic.y);
-          if (st < 3) {
-            inc[u][v] = ic;
-            if (tval[u]) {
-              const double sx = rho[u][v].x + cnext * (double)kr;
-              const double sy = rho[u][v].y + cnext * (double)ki;
-              sN[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = make_float2((float)sx, (float)sy);
-            }
-          } else {
-            rho[u][v].x += (double)ic.x;   // the state update itself is FP64
-            rho[u][v].y += (double)ic.y;
-          }
+          rho[u][v].x += (double)fmaf(cacc, ci[u][v], inc[u][v].x);
+          rho[u][v].y += (double)fmaf(cacc, -cr[u][v], inc[u][v].y);
         }
       }
       if (BRD) {
-        const float kr = bm, ki = -br;
-        float2 ic = (st == 0) ? make_float2(0.f, 0.f) : inc_b;
-        ic.x = fmaf(cacc, kr, ic.x);
-        ic.y = fmaf(cacc, ki, ic.y);
-        if (st < 3) {
-          inc_b = ic;
-          if (bwrite)
-            sN[bi * P + bj] = make_float2((float)(rho_b.x + cnext * (double)kr), (float)(rho_b.y + cnext * (double)ki));
-        } else {
-          rho_b.x += (double)ic.x;
-          rho_b.y += (double)ic.y;
-        }
-      }
-      if (st < 3) {
-        __syncthreads();  // S of the next stage is complete
-        ph ^= (int)DB;
-        continue;
+        rho_b.x += (double)fmaf(cacc, bm, inc_b.x);
+        rho_b.y += (double)fmaf(cacc, -br, inc_b.y);
       }
 
       // ---------------- end of step: optional trace renormalisation, records
@@ -496,13 +543,26 @@ lindblad_rk4_resident(ResidentArgs a) {
       }
 #pragma unroll
       for (int u = 0; u < BPW; ++u) {
-        if (!tval[u]) continue;
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          sN[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] =
-              make_float2((float)rho[u][v].x, (float)rho[u][v].y);
+        for (int v = 0; v < 4; ++v) {
+          const float2 sv = make_float2((float)rho[u][v].x, (float)rho[u][v].y);
+          if (tval[u]) sN[(trow[u] + 4 * lg + v) * P + tcol[u] + lr] = sv;
+          if constexpr (DFOLD) {
+            const float2 p = tval[u] ? cmulf(dc[u][v], sv) : make_float2(0.f, 0.f);
+            cr[u][v] = -p.y;
+            ci[u][v] = p.x;
+          }
+        }
       }
-      if (BRD && bwrite) sN[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
+      if (BRD) {
+        const float2 sv = make_float2((float)rho_b.x, (float)rho_b.y);
+        if (bwrite) sN[bi * P + bj] = sv;
+        if constexpr (DFOLD) {
+          const float2 p = bwrite ? cmulf(dc_b, sv) : make_float2(0.f, 0.f);
+          br = -p.y;
+          bm = p.x;
+        }
+      }
 
       const int g = a.step_offset + s + 1;
       const bool rec_now = record_any && (g % a.nout) == 0 && (g / a.nout - 1) < a.rec_capacity;
