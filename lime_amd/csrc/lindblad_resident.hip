@@ -573,23 +573,46 @@ lindblad_rk4_resident(ResidentArgs a) {
           for (int e = 0; e < a.n_eops; ++e) {
             const double2* E = E0 + (size_t)e * nn;
             double pr = 0.0, pi = 0.0;
-#pragma unroll
-            for (int u = 0; u < BPW; ++u) {
-              const int j = tcol[u] + lr;
+            if constexpr (CACHE && BPW == 1) {
+              // the owned elements of E^T (unless cached), loaded at clamped, always valid
+              // addresses so that the loads leave together; zero outside the operator
+              double2 ev[4];
+              const int j = tcol[0] + lr;
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
-                const int i = trow[u] + 4 * lg + v;
-                if (tval[u] && i < n && j < n) {
-                  double2 ev;
-                  if constexpr (ECACHE && REC) {
-                    ev = ecached ? ec[u][v] : E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
-                  } else {
-                    ev = E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
-                  }
-                  pr += ev.x * rho[u][v].x - ev.y * rho[u][v].y;
-                  pi += ev.x * rho[u][v].y + ev.y * rho[u][v].x;
+                const int i = trow[0] + 4 * lg + v;
+                const bool in = tval[0] && i < n && j < n;
+                const double2 x = E[in ? (trow[0] + v) * n + tcol[0] + 4 * olg * n + olr : 0];
+                ev[v] = in ? x : make_double2(0.0, 0.0);
+                if constexpr (ECACHE && REC) {
+                  if (ecached) ev[v] = ec[0][v];   // zero outside the operator
                 }
-                if (!CACHE) __builtin_amdgcn_sched_barrier(0);
+              }
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                pr += ev[v].x * rho[0][v].x - ev[v].y * rho[0][v].y;
+                pi += ev[v].x * rho[0][v].y + ev[v].y * rho[0][v].x;
+              }
+            } else {
+              // no register room: one load in flight
+#pragma unroll
+              for (int u = 0; u < BPW; ++u) {
+                const int j = tcol[u] + lr;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                  const int i = trow[u] + 4 * lg + v;
+                  if (tval[u] && i < n && j < n) {
+                    double2 x;
+                    if constexpr (ECACHE && REC) {
+                      x = ecached ? ec[u][v] : E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+                    } else {
+                      x = E[(trow[u] + v) * n + tcol[u] + 4 * olg * n + olr];
+                    }
+                    pr += x.x * rho[u][v].x - x.y * rho[u][v].y;
+                    pi += x.x * rho[u][v].y + x.y * rho[u][v].x;
+                  }
+                  if (!CACHE) __builtin_amdgcn_sched_barrier(0);
+                }
               }
             }
             if (BRD && bwrite) {
