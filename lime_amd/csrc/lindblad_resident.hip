@@ -113,9 +113,11 @@ lindblad_rk4_resident(ResidentArgs a) {
   constexpr int NT = TG * TG;
   static_assert(W * BPW >= NT, "every tile needs an owner");
   constexpr int NC = 16 * (NB - 1);          // bordered: core size = border index
-  constexpr int NBP = BRD ? 2 * NC + 1 : 0;  // bordered: number of border outputs
-  constexpr int BG = (T >= 4 * NBP) ? 4 : (T >= 2 * NBP) ? 2 : 1;
-  static_assert(!BRD || T >= NBP, "every border output needs an owner");
+  // bordered: BG lanes per border output, 2 NC groups of them (the corner rides with
+  // group 0); the groups fill whole waves (BG NC is a multiple of 32)
+  constexpr int BG = (T >= 8 * NC) ? 4 : 2;
+  static_assert(!BRD || T >= 4 * NC, "every border output needs an owner");
+  constexpr int BWAVES = BRD ? (2 * NC * BG) / 64 : 0;   // waves holding border groups
   static_assert(!(BRD && DENSE), "bordered kernels are built without dense jump operators");
   // per-lane cache of D (needs the register budget of one tile per lane at <= 4 waves per
   // SIMD, or of at most two waves per SIMD); of a single observable only at <= 2 waves per
@@ -182,24 +184,28 @@ lindblad_rk4_resident(ResidentArgs a) {
     trow[u] = (tt / TG) * 16;
     tcol[u] = (tt % TG) * 16;
   }
-  // ---- bordered: border output (bi, bj) of lane group tid / BG; lane bq of the group
-  // sums k = bq, bq + BG, ...; every lane of the group ends with the same FP64 state
-  int bi = 0, bj = 0, bq = 0;
+  // ---- bordered: lane group g = tid / BG (g < 2 NC) sums border output (gi, gj), its lane
+  // bq taking k = bq, bq + BG, ...; groups 0..NC-1 are the row outputs (NC, g), groups
+  // NC..2NC-1 the column outputs (g - NC, NC).  Group 0 also sums the corner (NC, NC).  The
+  // group's lane 0 owns its output's FP64 state (bi, bj) = (gi, gj); lane 1 owns the corner.
+  int gi = 0, gj = 0, bq = 0;
   bool bown = false;
   if (BRD) {
     const int grp = tid / BG;
     bq = tid % BG;
-    bown = grp < NBP;
+    bown = grp < 2 * NC;
     const int gg = bown ? grp : 0;
-    if (gg <= NC) {
-      bi = NC;
-      bj = gg;
+    if (gg < NC) {
+      gi = NC;
+      gj = gg;
     } else {
-      bi = gg - NC - 1;
-      bj = NC;
+      gi = gg - NC;
+      gj = NC;
     }
   }
-  const bool bwrite = bown && bq == 0;
+  const bool bcorner = BRD && tid == 1;
+  const bool bwrite = (bown && bq == 0) || bcorner;
+  const int bi = bcorner ? NC : gi, bj = bcorner ? NC : gj;
 
   double2 rho[BPW][4];   // FP64 state
   float2 inc[BPW][4];    // RK4 increment dt*(k1 + 2k2 + 2k3 + k4)/6, summed in FP32
@@ -258,11 +264,13 @@ lindblad_rk4_resident(ResidentArgs a) {
 
   // C accumulators of the stage.  DFOLD (D cached in registers): D o S is their initial
   // value, formed by the epilogue that wrote S -- every lane writes the S elements it owns,
-  // so no stage reads S back for the Hadamard term.  br, bm: C at the border output
+  // so no stage reads S back for the Hadamard term.  br, bm: C at the border output (the
+  // lane's partial sum until the group sum); dbr, dbm: the D o S part of the owned border
+  // output, added after the group sum
   // (D o S only on the group's lane 0, since the group's partial sums are added up).
   constexpr bool DFOLD = CACHE;
   floatx4 cr[BPW], ci[BPW];
-  float br = 0.f, bm = 0.f;
+  float br = 0.f, bm = 0.f, dbr = 0.f, dbm = 0.f;
   // the accumulators hold C and the stage derivative is k = -i C, so D o S enters C as
   // (Re C, Im C) = (-Im(D o S), Re(D o S))
   if constexpr (DFOLD) {
@@ -276,8 +284,8 @@ lindblad_rk4_resident(ResidentArgs a) {
       }
     if (BRD && bwrite) {
       const float2 p = cmulf(dc_b, make_float2((float)rho_b.x, (float)rho_b.y));
-      br = -p.y;
-      bm = p.x;
+      dbr = -p.y;
+      dbm = p.x;
     }
   }
 
@@ -301,9 +309,10 @@ lindblad_rk4_resident(ResidentArgs a) {
           cr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
           ci[u] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
-        br = 0.f;
-        bm = 0.f;
       }
+      br = 0.f;
+      bm = 0.f;
+      float qr = 0.f, qm = 0.f;   // bordered: the corner's partial sum (wave 0)
       // C += H_eff S  and  C -= S H_eff^dag  ((S H^dag)_ij = sum_k S_ik conj(H_jk)).
       // Software pipelined by hand: fragments of step k0+4 are read from LDS while the
       // MFMAs of step k0 issue, so at most two fragment sets are live.  Bordered: the
@@ -370,16 +379,25 @@ lindblad_rk4_resident(ResidentArgs a) {
             }
           }
           if constexpr (BRD) {
-            if (bown) {
+            // border terms k = k0 + bq + BG j (up to NC + 3: zero padding beyond NC); the
+            // border waves are whole waves, so these branches are wave-uniform
+            if (w < BWAVES) {
 #pragma unroll
               for (int j = 0; j < 4 / BG; ++j) {
                 const int k = k0 + bq + BG * j;
-                if (k <= NC) {
-                  const float2 hh = sH[bi * P + k], ss = sS[k * P + bj];
-                  const float2 zz = sS[bi * P + k], gg = sH[bj * P + k];
-                  br += hh.x * ss.x - hh.y * ss.y - (zz.x * gg.x + zz.y * gg.y);
-                  bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
-                }
+                const float2 hh = sH[gi * P + k], ss = sS[k * P + gj];
+                const float2 zz = sS[gi * P + k], gg = sH[gj * P + k];
+                br += hh.x * ss.x - hh.y * ss.y - (zz.x * gg.x + zz.y * gg.y);
+                bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
+              }
+            }
+            if (w == 0) {   // the corner, summed by every group of wave 0 alike
+#pragma unroll
+              for (int j = 0; j < 4 / BG; ++j) {
+                const int k = k0 + bq + BG * j;
+                const float2 hh = sH[NC * P + k], ss = sS[k * P + NC], zz = sS[NC * P + k];
+                qr += hh.x * ss.x - hh.y * ss.y - (zz.x * hh.x + zz.y * hh.y);
+                qm += hh.x * ss.y + hh.y * ss.x - (zz.y * hh.x - zz.x * hh.y);
               }
             }
           }
@@ -404,6 +422,16 @@ lindblad_rk4_resident(ResidentArgs a) {
       if (BRD) {
         br = group_sum<BG>(br);
         bm = group_sum<BG>(bm);
+        if (w == 0) {
+          qr = group_sum<BG>(qr);
+          qm = group_sum<BG>(qm);
+          if (bcorner) {
+            br = qr;
+            bm = qm;
+          }
+        }
+        br += dbr;
+        bm += dbm;
       }
       // C += i sum_k (L_k S) R_k^dag, every factor staged through LDS (sL) so the fragment
       // loops are the same pipelined LDS reads as the H_eff products.
@@ -494,8 +522,8 @@ lindblad_rk4_resident(ResidentArgs a) {
           if (bwrite) sN[bi * P + bj] = sv;
           if constexpr (DFOLD) {
             const float2 p = bwrite ? cmulf(dc_b, sv) : make_float2(0.f, 0.f);
-            br = -p.y;
-            bm = p.x;
+            dbr = -p.y;
+            dbm = p.x;
           }
         }
         __syncthreads();  // S of the next stage is complete
@@ -559,8 +587,8 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (bwrite) sN[bi * P + bj] = sv;
         if constexpr (DFOLD) {
           const float2 p = bwrite ? cmulf(dc_b, sv) : make_float2(0.f, 0.f);
-          br = -p.y;
-          bm = p.x;
+          dbr = -p.y;
+          dbm = p.x;
         }
       }
 
