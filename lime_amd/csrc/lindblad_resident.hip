@@ -8,7 +8,7 @@
 //   f(X) = -i (H_eff X - X H_eff^dag) + D o X + sum_k L_k X L_k^dag.
 //
 // Work decomposition (wave64, 16x16x4 FP32 MFMA tiles):
-//   * the operator is zero-padded to NP = 16*NB; its NB*NB output tiles of 16x16 are
+//   * the operator is zero-padded to NP = 16*NB (bordered layouts: below); its NB*NB output tiles of 16x16 are
 //     dealt to the W waves of the workgroup, tile t = w + u*W for u < BPW;
 //   * lane l owns, in each of its tiles, the 4 elements (4*(l/16)+v, l%16), v = 0..3 --
 //     the MFMA accumulator layout -- so the FP64 state (rho and the RK4 accumulator)
@@ -106,7 +106,9 @@ __device__ __forceinline__ void lds_gemm(const float2* sA, const float2* sB, con
 template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD, bool DB>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(MINW)))
 lindblad_rk4_resident(ResidentArgs a) {
-  constexpr int NP = NB * 16;
+  // LDS rows / columns held: bordered layouts hold the core, the border index and 7 zero
+  // ones (the reductions and the fragment prefetch read up to index 16 (NB - 1) + 7)
+  constexpr int NP = BRD ? 16 * (NB - 1) + 8 : NB * 16;
   constexpr int P = NP + 1;
   constexpr int T = W * 64;
   constexpr int TG = BRD ? NB - 1 : NB;     // MFMA tile grid edge
@@ -789,7 +791,7 @@ constexpr size_t kLdsMax = 160 * 1024;
 // prefetch spare rows) + the FP64 reduction slots
 size_t lds_layout(int n, int variant, int n_dense, int n_eops, int nbuf) {
   const int nb = cdiv(n, 16);
-  const int np = nb * 16, p = np + 1;
+  const int np = resident_bordered(n, n_dense) ? 16 * (nb - 1) + 8 : nb * 16, p = np + 1;
   const size_t buf = (size_t)np * p + 4 * p + 4;
   size_t bytes = (np * p + nbuf * buf) * sizeof(float2);
   bytes += (size_t)(1 + n_eops) * shape_of(nb, variant, resident_bordered(n, n_dense)).w * sizeof(double2);
