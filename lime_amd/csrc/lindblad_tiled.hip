@@ -200,33 +200,38 @@ constexpr int BP = 104;         // LDS pitch (BF16 slots) of a split row/column:
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ unsigned bf16_rne(float x) {        // round to nearest even
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
-__device__ __forceinline__ void split_bf16(float x, unsigned& hi, unsigned& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - __uint_as_float(hi << 16));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+// Split both parts of v into BF16 pairs x = hi + lo (each rounded to nearest even by
+// v_cvt_pk_bf16_f32; x - hi is exact in FP32).  Packed: h = re_hi | im_hi << 16,
+// l = re_lo | im_lo << 16.
+__device__ __forceinline__ void split_bf16(float2 v, unsigned& h, unsigned& l) {
+  const float2v x = {v.x, v.y};
+  const bf16x2 hb = __builtin_convertvector(x, bf16x2);
+  const float2v hf = __builtin_convertvector(hb, float2v);
+  const bf16x2 lb = __builtin_convertvector(x - hf, bf16x2);
+  h = __builtin_bit_cast(unsigned, hb);
+  l = __builtin_bit_cast(unsigned, lb);
 }
 
 __device__ __forceinline__ void put_a(unsigned short* row, int k, float2 v) {
-  unsigned rh, rl, ih, il;
-  split_bf16(v.x, rh, rl);
-  split_bf16(v.y, ih, il);
-  *reinterpret_cast<uint2*>(row + 4 * k) = make_uint2(rh | (rl << 16), ih | (il << 16));
-  *reinterpret_cast<unsigned*>(row + 64 + 2 * k) = rh | (ih << 16);
+  unsigned h, l;
+  split_bf16(v, h, l);
+  // (ar_hi, ar_lo, ai_hi, ai_lo) | (ar_hi, ai_hi)
+  *reinterpret_cast<uint2*>(row + 4 * k) = make_uint2((h & 0xffffu) | (l << 16), (h >> 16) | (l & 0xffff0000u));
+  *reinterpret_cast<unsigned*>(row + 64 + 2 * k) = h;
 }
 
 __device__ __forceinline__ void put_b(unsigned short* colr, unsigned short* coli, int k, float2 v) {
-  unsigned rh, rl, ih, il;
-  split_bf16(v.x, rh, rl);
-  split_bf16(v.y, ih, il);
-  const unsigned nih = ih ^ 0x8000u, nil = il ^ 0x8000u;
+  unsigned h, l;
+  split_bf16(v, h, l);
+  const unsigned rh = h & 0xffffu, ih = h >> 16, nih = ih ^ 0x8000u;
+  // Br: (br_hi, br_hi, -bi_hi, -bi_hi) | (br_lo, -bi_lo);  Bi: (bi_hi, bi_hi, br_hi, br_hi) | (bi_lo, br_lo)
   *reinterpret_cast<uint2*>(colr + 4 * k) = make_uint2(rh | (rh << 16), nih | (nih << 16));
-  *reinterpret_cast<unsigned*>(colr + 64 + 2 * k) = rl | (nil << 16);
+  *reinterpret_cast<unsigned*>(colr + 64 + 2 * k) = l ^ 0x80000000u;
   *reinterpret_cast<uint2*>(coli + 4 * k) = make_uint2(ih | (ih << 16), rh | (rh << 16));
-  *reinterpret_cast<unsigned*>(coli + 64 + 2 * k) = il | (rl << 16);
+  *reinterpret_cast<unsigned*>(coli + 64 + 2 * k) = (l >> 16) | (l << 16);
 }
 
 // B element of a product term from the source operand: PLAIN b = M[k][j];
