@@ -108,19 +108,20 @@ class ChainQRT:
     def _segment_states(self, c0, length):
         """Recompute steps c0 .. c0+length-1 from the checkpoint at step c0.
 
-        Returns X (B, length, N, N) with X[:, j] = X_{c0+j} and the FP64 stage operands
-        (length, 3, B, N, N) of the step leaving X_{c0+j}: X + h/2 k1, X + h/2 k2, X + h k3
-        -- the values the forward step used, recorded by the per-stage kernels."""
+        Returns X (length, B, N, N) with X[j] = X_{c0+j} (record-major, so each step's
+        states are one contiguous block) and the FP64 stage operands (length, 3, B, N, N) of
+        the step leaving X_{c0+j}: X + h/2 k1, X + h/2 k2, X + h k3 -- the values the forward
+        step used, recorded by the per-stage kernels."""
         B, n = self.B, self.n
         Xc = self.x0.clone() if c0 == 0 else self.checkpoints[:, c0 // self.K - 1].clone().contiguous()
-        out = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
-        out[:, 0] = Xc
+        out = torch.empty((length, B, n, n), dtype=torch.complex128, device=self.dev)
+        out[0] = Xc
         stages = torch.empty((length, 3, B, n, n), dtype=torch.complex128, device=self.dev)
         st = torch.empty((B, length, n, n), dtype=torch.complex128, device=self.dev)
         self.rec_prop.step(Xc, self.dt, length, states=st, stage_states=stages, nout=1,
                            step_offset=0, rec_capacity=length)
         if length > 1:
-            out[:, 1:] = st[:, :length - 1]
+            out[1:] = st[:, :length - 1].transpose(0, 1)
         return out, stages
 
     def backward(self, G):
@@ -128,7 +129,7 @@ class ChainQRT:
         B, n, h = self.B, self.n, self.dt
         dev = self.dev
         mu_h = self.mu.conj().transpose(0, 1).contiguous()
-        G = G.to(dev, torch.complex128)
+        Gt = G.to(dev, torch.complex128).t().contiguous()     # (Nt, B): Gt[k] contiguous
         lam = torch.zeros((B, n, n), dtype=torch.complex128, device=dev)
         outa = torch.zeros((B, n), dtype=torch.complex128, device=dev)
         outb = torch.zeros((B, n), dtype=torch.complex128, device=dev)
@@ -148,8 +149,8 @@ class ChainQRT:
             Xs, stages = self._segment_states(c0, length)
             for k in range(c0 + length - 1, c0 - 1, -1):
                 # lambda' = lambda_{k+1}: add the source of record k+1 (C_{k+1} = Tr[mu X_{k+1}])
-                lam.add_(G[:, k].reshape(B, 1, 1) * mu_h)
-                X = Xs[:, k - c0].contiguous()
+                lam.addcmul_(Gt[k].view(B, 1, 1), mu_h)
+                X = Xs[k - c0]
                 s2, s3, s4 = stages[k - c0]                      # contiguous views
                 k4 = (h / 6.0) * lam
                 k4l = k4.to(opd).contiguous()
