@@ -323,11 +323,13 @@ def test_tiled_trace_renormalisation_records_after_scaling(n, prec):
     assert err < tol
 
 
-@pytest.mark.parametrize("n,n_e", [(33, 1), (65, 1), (65, 2), (48, 1)])
+@pytest.mark.parametrize("n,n_e", [(33, 1), (65, 1), (65, 2), (48, 1), (65, 65), (65, 130)])
 def test_qrt_per_operator_observables(n, n_e):
     """One observable set per operator (eop_group = 1), as in the absorption ensemble: a
     single observable is held in registers for the whole launch, several are read from
-    memory at every record; both against the oracle."""
+    memory at every record; both against the oracle.  At n = 65 the record slots of 65
+    observables still fit beside the double-buffered stage operand, those of 130 do not
+    (single-buffered kernel)."""
     rng = np.random.default_rng(300 + n + n_e)
     S, G = 2, 3
     Hs = np.stack([rand_herm(rng, n) for _ in range(S)])
