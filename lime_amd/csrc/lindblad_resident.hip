@@ -672,9 +672,11 @@ lindblad_rk4_resident(ResidentArgs a) {
       }
       __syncthreads();  // S of the next step (and the record partials) are complete
       ph ^= (int)DB;
-      if (rec_now && a.obs != nullptr) {
+      // the record's final sums go to the last wave: in the bordered kernels it holds no
+      // border group, so it reaches the stage barrier first and has the slack
+      if (rec_now && a.obs != nullptr && w == W - 1) {
         const int rec = g / a.nout - 1;
-        for (int e = tid; e < a.n_eops; e += T) {
+        for (int e = lane; e < a.n_eops; e += 64) {
           double2 t = make_double2(0.0, 0.0);
           for (int x = 0; x < W; ++x) {
             const double2 y = sRed[W + e * W + x];
