@@ -188,8 +188,10 @@ lindblad_rk4_resident(ResidentArgs a) {
   }
   // ---- bordered: lane group g = tid / BG (g < 2 NC) sums border output (gi, gj), its lane
   // bq taking k = bq, bq + BG, ...; groups 0..NC-1 are the row outputs (NC, g), groups
-  // NC..2NC-1 the column outputs (g - NC, NC).  Group 0 also sums the corner (NC, NC).  The
-  // group's lane 0 owns its output's FP64 state (bi, bj) = (gi, gj); lane 1 owns the corner.
+  // NC..2NC-1 the column outputs (g - NC, NC).  The group's lane 0 owns its output's FP64
+  // state (bi, bj) = (gi, gj).  The corner (NC, NC) is summed by the first wave without
+  // border groups (CW = BWAVES, which has slack at the stage barrier; its lane 0 owns it),
+  // or, when every wave holds groups, by the groups of wave 0 (lane 1 owns it).
   int gi = 0, gj = 0, bq = 0;
   bool bown = false;
   if (BRD) {
@@ -205,7 +207,8 @@ lindblad_rk4_resident(ResidentArgs a) {
       gj = NC;
     }
   }
-  const bool bcorner = BRD && tid == 1;
+  constexpr int CW = (W > BWAVES) ? BWAVES : 0;   // wave summing the corner
+  const bool bcorner = BRD && tid == (CW > 0 ? 64 * CW : 1);
   const bool bwrite = (bown && bq == 0) || bcorner;
   const int bi = bcorner ? NC : gi, bj = bcorner ? NC : gj;
 
@@ -228,7 +231,7 @@ lindblad_rk4_resident(ResidentArgs a) {
         if (tval[u]) sS0[i * P + j] = make_float2((float)x.x, (float)x.y);
       }
     }
-    if (bown) {
+    if (bown || bcorner) {
       rho_b = R[bi * n + bj];
       if (bwrite) sS0[bi * P + bj] = make_float2((float)rho_b.x, (float)rho_b.y);
     }
@@ -254,11 +257,11 @@ lindblad_rk4_resident(ResidentArgs a) {
         if constexpr (ECACHE && REC) ec[u][v] = (ecached && in) ? E[i * n + j] : make_double2(0.0, 0.0);
       }
     }
-    if (bown) {
+    if (bown || bcorner) {
       if (has_d) dc_b = Dm[bi * n + bj];
       if (ecached) ec_b = E[bi * n + bj];
     }
-  } else if (BRD && bown) {
+  } else if (BRD && (bown || bcorner)) {
     if (has_d) dc_b = a.dmat[(size_t)set * nn + bi * n + bj];
     if (REC && ecached) ec_b = a.eops_t[(size_t)(b / a.eop_group) * a.n_eops * nn + bi * n + bj];
   }
@@ -393,7 +396,7 @@ lindblad_rk4_resident(ResidentArgs a) {
                 bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
               }
             }
-            if (w == 0) {   // the corner, summed by every group of wave 0 alike
+            if (w == CW) {   // the corner, summed by every lane group of wave CW alike
 #pragma unroll
               for (int j = 0; j < 4 / BG; ++j) {
                 const int k = k0 + bq + BG * j;
@@ -424,7 +427,7 @@ lindblad_rk4_resident(ResidentArgs a) {
       if (BRD) {
         br = group_sum<BG>(br);
         bm = group_sum<BG>(bm);
-        if (w == 0) {
+        if (w == CW) {
           qr = group_sum<BG>(qr);
           qm = group_sum<BG>(qm);
           if (bcorner) {
