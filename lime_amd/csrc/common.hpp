@@ -31,24 +31,13 @@ __device__ __forceinline__ double dpp_f64(double x) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Sum over the lanes of each aligned group of G lanes (G = 1, 2, 4 or 8); every lane of
-// the group receives the group's sum (at G = 8, after the quad steps every lane of a quad
-// holds the quad's sum, so the mirrored lane of the other quad holds the other partial).
+// Sum over the lanes of each aligned group of G lanes (G = 1, 2 or 4); every lane of the
+// group receives the group's sum.
 template <int G>
 __device__ __forceinline__ float group_sum(float x) {
-  static_assert(G == 1 || G == 2 || G == 4 || G == 8, "groups of 1, 2, 4 or 8 lanes");
+  static_assert(G == 1 || G == 2 || G == 4, "groups of 1, 2 or 4 lanes");
   if constexpr (G >= 2) x += dpp_f32<0xB1>(x);
   if constexpr (G >= 4) x += dpp_f32<0x4E>(x);
-  if constexpr (G >= 8) x += dpp_f32<0x141>(x);
-  return x;
-}
-
-// Sum over the 64 lanes of the wave (FP32), returned to every lane.
-__device__ __forceinline__ float wave_sum_f32(float x) {
-  x = group_sum<8>(x);
-  x += dpp_f32<0x140>(x);
-  x += __shfl_xor(x, 16, 64);
-  x += __shfl_xor(x, 32, 64);
   return x;
 }
 
