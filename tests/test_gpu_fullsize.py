@@ -1,0 +1,91 @@
+"""The headline workload at its full size: 4096 disorder realisations of the 8 x 8 exciton
+lattice (n = 65), 3 operators each = 12288 operators in one launch.
+
+The oracle cannot propagate 12288 operators in a test, so the full batch is checked through
+properties that do not depend on its size, plus oracle spot checks of single operators:
+
+* linearity: the operator X_x + 2 X_y propagates to rho_x + 2 rho_y (FP32 products: 1e-5
+  relative);
+* position independence: an operator propagated inside the full batch equals, bit for bit,
+  the same operator propagated alone (one workgroup per operator, no cross-talk);
+* the FP64 oracle on realisations 0 and 4095 (C(t) 1e-6 relative over 256 steps);
+* density matrices of every realisation keep Tr rho = 1 and rho = rho^dag (no
+  renormalisation: the generator is trace-free, so only rounding moves the trace).
+"""
+import numpy as np
+import pytest
+
+from oracle import lindblad as ol
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+NT = 256
+
+
+@pytest.fixture(scope="module")
+def lattice():
+    from lime_amd import configs
+    return configs.lattice_spectrum(seed=2, n_real=4096)
+
+
+def test_full_batch_linearity_position_and_oracle(lattice):
+    from lime_amd.propagator import LindbladPropagator
+    cfg = lattice
+    R, n, dt = 4096, cfg["n"], cfg["dt"]
+    x0 = cfg["x0"].copy()
+    x0[:, 2] = x0[:, 0] + 2.0 * x0[:, 1]                  # third operator: X_x + 2 X_y
+    dev = torch.device("cuda", 0)
+    prop = LindbladPropagator(cfg["H"], cfg["c_ops"], group=3, variant=0)
+    X = torch.as_tensor(x0.reshape(3 * R, n, n), device=dev).contiguous()
+    E_t = torch.as_tensor(cfg["mu"].reshape(3 * R, 1, n, n), device=dev).transpose(-1, -2).contiguous()
+    obs = torch.zeros((3 * R, NT, 1), dtype=torch.complex128, device=dev)
+    print(f"[fullsize] 12288 operators, n={n}, {NT} steps", flush=True)
+    prop.step(X, dt, NT, eops_t=E_t, eop_group=1, obs=obs, nout=1, rec_capacity=NT)
+    torch.cuda.synchronize()
+    rho = X.reshape(R, 3, n, n)
+    lin = rho[:, 0] + 2.0 * rho[:, 1]
+    scale = float(torch.max(torch.abs(lin)))
+    err = float(torch.max(torch.abs(rho[:, 2] - lin)))
+    print(f"[fullsize] linearity max error {err:.3e} (scale {scale:.3e})", flush=True)
+    assert err < 1e-5 * scale
+
+    # the last realisation alone, same kernel variant: bitwise equal
+    r = R - 1
+    solo = LindbladPropagator(cfg["H"][r:r + 1], cfg["c_ops"], group=3, variant=0)
+    Xs = torch.as_tensor(x0[r], device=dev).contiguous()
+    Es = torch.as_tensor(cfg["mu"][r].reshape(3, 1, n, n), device=dev).transpose(-1, -2).contiguous()
+    os_ = torch.zeros((3, NT, 1), dtype=torch.complex128, device=dev)
+    solo.step(Xs, dt, NT, eops_t=Es, eop_group=1, obs=os_, nout=1, rec_capacity=NT)
+    torch.cuda.synchronize()
+    assert torch.equal(Xs, rho[r])
+    assert torch.equal(os_, obs.reshape(R, 3, NT, 1)[r])
+
+    # FP64 oracle on the first and the last realisation (component x)
+    C = obs.reshape(R, 3, NT).cpu().numpy()
+    for real in (0, R - 1):
+        _, ref, _ = ol.propagate(cfg["H"][real], cfg["x0"][real, 0], cfg["c_ops"], dt, NT,
+                                 [cfg["mu"][real, 0]])
+        rel = np.max(np.abs(C[real, 0] - ref[:, 0])) / np.max(np.abs(ref))
+        print(f"[fullsize] realisation {real}: C(t) max rel error {rel:.3e}", flush=True)
+        assert rel < 1e-6
+
+
+def test_full_batch_density_matrices_keep_trace_and_hermiticity(lattice):
+    from lime_amd.propagator import LindbladPropagator
+    cfg = lattice
+    R, n, dt = 4096, cfg["n"], cfg["dt"]
+    v = cfg["mu"][:, 0, :, 0]                              # (R, n): mu_x |g> per realisation
+    rho0 = np.einsum("ri,rj->rij", v, v.conj()) / np.einsum("ri,ri->r", v.conj(), v).real[:, None, None]
+    dev = torch.device("cuda", 0)
+    prop = LindbladPropagator(cfg["H"], cfg["c_ops"], variant=0)
+    X = torch.as_tensor(rho0, device=dev).contiguous()
+    print(f"[fullsize] {R} density matrices, n={n}, {NT} steps", flush=True)
+    prop.step(X, dt, NT)
+    torch.cuda.synchronize()
+    tr = torch.diagonal(X, dim1=-2, dim2=-1).sum(-1)
+    herm = float(torch.max(torch.abs(X - X.conj().transpose(-1, -2))))
+    terr = float(torch.max(torch.abs(tr - 1)))
+    print(f"[fullsize] trace residual {terr:.3e}, hermiticity residual {herm:.3e}", flush=True)
+    assert terr < 1e-6
+    assert herm < 1e-7
