@@ -1,6 +1,7 @@
 """Workloads at their full size.  The headline: 4096 disorder realisations of the 8 x 8
-exciton lattice (n = 65), 3 operators each = 12288 operators in one launch; and the
-n = 512 system with 16 dense jump operators at batch 32 (FP32 and 3xBF16).
+exciton lattice (n = 65), 3 operators each = 12288 operators in one launch; the n = 512
+system with 16 dense jump operators at batch 32 (FP32 and 3xBF16); the adjoint gradient of
+256 chains over 2000 steps.
 
 The oracle cannot propagate 12288 operators in a test, so the full batch is checked through
 properties that do not depend on its size, plus oracle spot checks of single operators:
@@ -125,3 +126,60 @@ def test_dense512_full_batch(prec, tol):
     err0 = np.max(np.abs(out["fp64"][0].cpu().numpy() - ref)) / np.max(np.abs(ref))
     print(f"[fullsize] fp64 GPU vs oracle (operator 0): {err0:.3e}", flush=True)
     assert err0 < 1e-12
+
+
+def test_gradient_full_batch():
+    """The gradient workload at full size (256 chains of 128 sites + ground state, n = 129,
+    2000 RK4 steps, checkpoints every 64): the FP32 adjoint gradient against the FP64 adjoint
+    gradient of the same batch, and the FP64 adjoint against central differences of the FP64
+    forward on two site energies (h = 1e-4) and the shared dephasing rate (h = 1e-5: at
+    1e-4 = 0.5 % of gamma the difference quotient's own O(h^2) error reaches ~1e-5)."""
+    from lime_amd import configs
+    from lime_amd.adjoint import ChainAbsorption, spectrum_loss
+    cfg = configs.chain_gradient()
+    tgt = configs.chain_gradient(seed=cfg["seed"] + 1000)
+    dev = torch.device("cuda")
+    nt, K, dt = cfg["nt"], cfg["checkpoint"], cfg["dt"]
+    mu = torch.tensor(cfg["mu"][:, 0], device=dev)
+    gam = lambda: torch.tensor(cfg["gamma"], dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        target = ChainAbsorption.apply(torch.tensor(tgt["eps"], device=dev), torch.tensor(tgt["J"], device=dev),
+                                       gam(), mu, dt, nt, K, "fp64")
+
+    def loss(eps, J, gamma, prec):
+        return spectrum_loss(ChainAbsorption.apply(eps, J, gamma, mu, dt, nt, K, prec), target, dt).sum()
+
+    grads = {}
+    for prec in ("fp32", "fp64"):
+        eps = torch.tensor(cfg["eps"], device=dev, requires_grad=True)
+        J = torch.tensor(cfg["J"], device=dev, requires_grad=True)
+        gamma = gam().requires_grad_()
+        print(f"[fullsize] gradient, 256 chains x 2000 steps, {prec}", flush=True)
+        loss(eps, J, gamma, prec).backward()
+        grads[prec] = {"eps": eps.grad.cpu().numpy(), "J": J.grad.cpu().numpy(), "gamma": gamma.grad.item()}
+    for name in ("eps", "J"):
+        a, r = grads["fp32"][name], grads["fp64"][name]
+        rel = np.max(np.abs(a - r)) / np.max(np.abs(r))
+        print(f"[fullsize] {name}: fp32 vs fp64 adjoint max rel error {rel:.2e}", flush=True)
+        assert rel < 1e-4
+    relg = abs(grads["fp32"]["gamma"] - grads["fp64"]["gamma"]) / abs(grads["fp64"]["gamma"])
+    print(f"[fullsize] gamma: fp32 vs fp64 adjoint rel error {relg:.2e}", flush=True)
+    assert relg < 1e-4
+    with torch.no_grad():
+        for name, ix in (("eps", (0, 5)), ("eps", (255, 100)), ("gamma", ())):
+            h = 1e-5 if name == "gamma" else 1e-4
+            vals = []
+            for sgn in (1, -1):
+                eps = torch.tensor(cfg["eps"], device=dev)
+                J = torch.tensor(cfg["J"], device=dev)
+                gamma = gam()
+                if name == "eps":
+                    eps[ix] += sgn * h
+                else:
+                    gamma += sgn * h
+                vals.append(loss(eps, J, gamma, "fp64").item())
+            fd = (vals[0] - vals[1]) / (2 * h)
+            ad = grads["fp64"][name][ix] if name == "eps" else grads["fp64"]["gamma"]
+            scale = np.max(np.abs(grads["fp64"]["eps"])) if name == "eps" else abs(ad)
+            print(f"[fullsize] {name}{ix}: adjoint {ad:.6e}, central difference {fd:.6e}", flush=True)
+            assert abs(ad - fd) < 1e-5 * scale
