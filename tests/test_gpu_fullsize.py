@@ -1,7 +1,7 @@
 """Workloads at their full size.  The headline: 4096 disorder realisations of the 8 x 8
 exciton lattice (n = 65), 3 operators each = 12288 operators in one launch; the n = 512
 system with 16 dense jump operators at batch 32 (FP32 and 3xBF16); the adjoint gradient of
-256 chains over 2000 steps.
+256 chains over 2000 steps; the ragged batch of 2048 instances in one captured graph.
 
 The oracle cannot propagate 12288 operators in a test, so the full batch is checked through
 properties that do not depend on its size, plus oracle spot checks of single operators:
@@ -183,3 +183,42 @@ def test_gradient_full_batch():
             scale = np.max(np.abs(grads["fp64"]["eps"])) if name == "eps" else abs(ad)
             print(f"[fullsize] {name}{ix}: adjoint {ad:.6e}, central difference {fd:.6e}", flush=True)
             assert abs(ad - fd) < 1e-5 * scale
+
+
+def test_ragged_full_batch():
+    """The ragged workload at full size (2048 instances, n in 16..256, 4 dense L each, one
+    captured graph of 5 steps): two graph replays equal eager launches bit for bit, every
+    density matrix keeps Tr rho = 1 and rho = rho^dag, and the smallest, a middle and the
+    largest instance match the FP64 oracle."""
+    from lime_amd import configs
+    from lime_amd.ragged import RaggedLindblad
+    cfg = configs.ragged_gue()
+    inst = cfg["instances"]
+    dt, chunk = cfg["dt"], 5
+    print(f"[fullsize] ragged: {len(inst)} instances", flush=True)
+    graph = RaggedLindblad(inst, tune=False)
+    eager = RaggedLindblad(inst, tune=False)
+    graph.capture(dt, chunk)
+    for _ in range(2):
+        graph.graph.replay()
+        eager.step_eager(dt, chunk)
+        torch.cuda.synchronize()
+    for cg, ce in zip(graph.classes, eager.classes):
+        assert torch.equal(cg["rho"], ce["rho"]), f"n={cg['n']}"
+    worst_t, worst_h = 0.0, 0.0
+    for c in graph.classes:
+        r = c["rho"]
+        tr = torch.diagonal(r, dim1=-2, dim2=-1).sum(-1)
+        worst_t = max(worst_t, float(torch.max(torch.abs(tr - 1))))
+        worst_h = max(worst_h, float(torch.max(torch.abs(r - r.conj().transpose(-1, -2)))))
+    print(f"[fullsize] ragged trace residual {worst_t:.3e}, hermiticity {worst_h:.3e}", flush=True)
+    assert worst_t < 1e-6 and worst_h < 1e-7
+    got = graph.states()
+    sizes = cfg["sizes"]
+    for i in (int(np.argmin(sizes)), int(np.argmax(sizes == 128)), int(np.argmax(sizes))):
+        n = inst[i]["H"].shape[0]
+        c_ops = list(inst[i]["c_ops"]) + [np.sqrt(inst[i]["dephasing"]) * np.diag(np.eye(n)[k]) for k in range(n)]
+        ref, _, _ = ol.propagate(inst[i]["H"], inst[i]["rho0"], c_ops, dt, 2 * chunk)
+        err = np.max(np.abs(got[i] - ref))
+        print(f"[fullsize] ragged instance {i} (n={n}): max error {err:.3e}", flush=True)
+        assert err < 2e-7
