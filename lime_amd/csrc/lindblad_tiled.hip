@@ -709,12 +709,16 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
 //   out_u[j] += w (sum_m conj(k_jm) s_j+1,m + conj(k_j+1,m) s_jm
 //                  - sum_m conj(k_m,j+1) s_mj + conj(k_mj) s_m,j+1)   -> d/dH_j,j+1 (times -i)
 // One 1024-thread workgroup per operator, every global read coalesced: a column sweep
-// (thread per column, the rows split into CQ quarters whose partial sums meet in LDS in a
-// fixed order) gives the column sums, the diagonal and the column-pair terms; a row sweep
-// (wave per row, lanes along the row, FP64 wave reductions) the row sums and row-pair
-// terms.  Deterministic: no atomics.
+// (thread per column, the rows split into cq = 1024 / n bands (at most 8) whose partial
+// sums meet in LDS in a fixed order) gives the column sums, the diagonal and the
+// column-pair terms; a row sweep (wave per row, lanes along the row, FP64 wave reductions)
+// the row sums and row-pair terms.  Deterministic: no atomics.
 constexpr int CT_THREADS = 1024;
-constexpr int CQ = CT_THREADS / 256;
+constexpr int CQ_MAX = 8;
+__host__ __device__ inline int contract_bands(int n) {
+  const int q = CT_THREADS / (n > 0 ? n : 1);
+  return q < 1 ? 1 : (q > CQ_MAX ? CQ_MAX : q);
+}
 
 __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   extern __shared__ __attribute__((aligned(16))) unsigned char craw[];
@@ -722,14 +726,16 @@ __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   const size_t nn = (size_t)n * n;
   const double2* K = c.kappa + (size_t)b * nn;
   const double2* S = c.s + (size_t)b * nn;
+  const int CQ = contract_bands(n);
+  const int TQ = CT_THREADS / CQ;                    // threads per band (>= n when n <= 1024)
   double2* cs = reinterpret_cast<double2*>(craw);   // [CQ][n] column-sum partials
   double2* cp = cs + CQ * n;                         // [CQ][n] column-pair partials
   double2* dg = cp + CQ * n;                         // diagonal
   double2* rs = dg + n;                              // row sums
   double2* rp = rs + n;                              // row-pair terms
-  const int q = threadIdx.x / 256, tq = threadIdx.x % 256;
-  const int rows = (n + CQ - 1) / CQ, m0 = q * rows, m1 = min(n, m0 + rows);
-  for (int j = tq; j < n; j += 256) {
+  const int q = threadIdx.x / TQ, tq = threadIdx.x % TQ;
+  const int rows = (n + CQ - 1) / CQ, m0 = q * rows, m1 = q < CQ ? min(n, m0 + rows) : m0;
+  for (int j = tq; j < n && q < CQ; j += TQ) {
     double sr = 0, si = 0, pr = 0, pi = 0;
     for (int m = m0; m < m1; ++m) {
       const double2 k = K[(size_t)m * n + j], x = S[(size_t)m * n + j];
@@ -794,7 +800,7 @@ __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   }
 }
 
-size_t contract_lds_bytes(int n) { return (size_t)(2 * CQ + 3) * n * sizeof(double2); }
+size_t contract_lds_bytes(int n) { return (size_t)(2 * contract_bands(n) + 3) * n * sizeof(double2); }
 
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
   const size_t lds = contract_lds_bytes(c.n);
