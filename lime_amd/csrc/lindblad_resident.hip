@@ -348,19 +348,44 @@ lindblad_rk4_resident(ResidentArgs a) {
           dr[u] = floatx4{0.f, 0.f, 0.f, 0.f};
           di[u] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
+        // bordered: the core's k = NC term, H[i][NC] S[NC][j] - S[i][NC] conj(H[j][NC]), on
+        // VALU into the accumulators, so the MFMA k-steps stop at NC instead of issuing a
+        // fifth-of-a-step of border index and zero padding (k = NC..NC+3)
+        if constexpr (BRD) {
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            const float2 sj = sS[NC * P + tcol[u] + lr], gj = sH[(tcol[u] + lr) * P + NC];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int i = trow[u] + 4 * lg + v;
+              const float2 hi = sH[i * P + NC], zi = sS[i * P + NC];
+              cr[u][v] += hi.x * sj.x - hi.y * sj.y - (zi.x * gj.x + zi.y * gj.y);
+              ci[u][v] += hi.x * sj.y + hi.y * sj.x - (zi.y * gj.x - zi.x * gj.y);
+            }
+          }
+        }
 #pragma unroll 2
         for (int k0 = 0; k0 < ke; k0 += 4) {
+          // bordered: MFMA steps k0 < NC only; the step k0 = NC carries the border terms
+          const bool mstep = !BRD || k0 < NC;
           float2 xn[BPW], yn[BPW], zn[BPW], hn[BPW];
           // prefetch step k0+4; at k0+4 == ke this reads padding or spare rows, discarded
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
-            xn[u] = sH[oR[u] + k0 + 4];
-            yn[u] = sS[oC[u] + (k0 + 4) * P];
-            zn[u] = sS[oR[u] + k0 + 4];
-            hn[u] = sH[oHc[u] + k0 + 4];
+            if (!BRD || k0 + 4 < NC) {
+              xn[u] = sH[oR[u] + k0 + 4];
+              yn[u] = sS[oC[u] + (k0 + 4) * P];
+              zn[u] = sS[oR[u] + k0 + 4];
+              hn[u] = sH[oHc[u] + k0 + 4];
+            } else {
+              xn[u] = x[u];
+              yn[u] = y[u];
+              zn[u] = z[u];
+              hn[u] = h[u];
+            }
           }
 #pragma unroll
-          for (int u = 0; u < BPW; ++u) {
+          for (int u = 0; u < BPW && mstep; ++u) {
             if constexpr (SPLIT) {
               // Re C = Hx Sx - (Hy Sy + Sx Hx' + Sy Hy'),  Im C = Hx Sy + Hy Sx + Sx Hy' - Sy Hx'
               // (H' = H[j][k] of the S H^dag product)
