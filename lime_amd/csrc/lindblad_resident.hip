@@ -704,7 +704,11 @@ lindblad_rk4_resident(ResidentArgs a) {
       // border group, so it reaches the stage barrier first and has the slack
       if (rec_now && a.obs != nullptr && w == W - 1) {
         const int rec = g / a.nout - 1;
-        for (int e = lane; e < a.n_eops; e += 64) {
+        // the lane's record address is formed here, not hoisted into a register (or a
+        // scratch spill) held across the whole step
+        int e0 = lane;
+        asm volatile("" : "+v"(e0));
+        for (int e = e0; e < a.n_eops; e += 64) {
           double2 t = make_double2(0.0, 0.0);
           for (int x = 0; x < W; ++x) {
             const double2 y = sRed[W + e * W + x];
