@@ -231,7 +231,14 @@ def main():
                      "traffic_source": "profiles/lindblad_rk4_resident_lattice65_traffic.txt "
                                        "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": f"lindblad_rk4_resident (bordered n=65, variant {prop.variant})",
-                     "launch_ms_mean": float(np.mean(launch_ms))},
+                     "launch_ms_mean": float(np.mean(launch_ms)),
+                     # FP32 MFMA work the kernel issues: the two complex products in the
+                     # three-multiplication form over the nc = n - 1 core, 6 real products
+                     # of nc^3 per stage = 48 nc^3 per rho-step (the border row and column
+                     # run on VALU), against the canonical 64 n^3
+                     "issued_mfma_tflops": achieved * 48.0 * (n - 1) ** 3 / (64.0 * n ** 3),
+                     "issued_mfma_frac": achieved * 48.0 * (n - 1) ** 3 / (64.0 * n ** 3)
+                     / PEAK_FP32_MFMA_TFLOPS},
         "spectrum_peak": peak,
     }
     result.update(extra)

@@ -131,6 +131,13 @@ lindblad_rk4_resident(ResidentArgs a) {
   // sign-flip VALU instructions (+8 % at n = 65).  Built where the register budget
   // (512 / MINW VGPRs) covers ~56 VGPRs per tile + 72 of fixed state.
   constexpr bool SPLIT = 512 / MINW >= 56 * BPW + 72;
+  // G3 (with SPLIT, no dense operators): the two complex products take 3 real products each
+  // (Gauss), 6 MFMAs per k-step instead of 8, into three accumulators
+  //   R1 = Hx Sx - Sx Hx',  R2 = Hy Sy + Sy Hy',  M1 = (Hx + Hy)(Sx + Sy) - (Sx + Sy)(Hx' - Hy')
+  // (H' = H[j][k] of the S H^dag product), held in cr, dr, ci:
+  //   Re C = R1 - R2,  Im C = M1 - R1 - R2.
+  // A value (cr0, ci0) of C enters them as R1 = cr0, R2 = 0, M1 = ci0 + cr0.
+  constexpr bool G3 = SPLIT && !DENSE;
   // DB: double-buffered stage operand (when it fits the LDS; never with dense operators):
   // the epilogue writes the next S into the other buffer, so a stage needs one barrier
   static_assert(!(DB && DENSE), "the double buffer is built without dense operators");
@@ -285,7 +292,7 @@ lindblad_rk4_resident(ResidentArgs a) {
       for (int v = 0; v < 4; ++v) {
         const float2 p = cmulf(dc[u][v], make_float2((float)rho[u][v].x, (float)rho[u][v].y));
         cr[u][v] = -p.y;
-        ci[u][v] = p.x;
+        ci[u][v] = G3 ? p.x - p.y : p.x;
       }
     if (BRD && bwrite) {
       const float2 p = cmulf(dc_b, make_float2((float)rho_b.x, (float)rho_b.y));
@@ -359,8 +366,10 @@ lindblad_rk4_resident(ResidentArgs a) {
             for (int v = 0; v < 4; ++v) {
               const int i = trow[u] + 4 * lg + v;
               const float2 hi = sH[i * P + NC], zi = sS[i * P + NC];
-              cr[u][v] += hi.x * sj.x - hi.y * sj.y - (zi.x * gj.x + zi.y * gj.y);
-              ci[u][v] += hi.x * sj.y + hi.y * sj.x - (zi.y * gj.x - zi.x * gj.y);
+              const float tr = hi.x * sj.x - hi.y * sj.y - (zi.x * gj.x + zi.y * gj.y);
+              const float ti = hi.x * sj.y + hi.y * sj.x - (zi.y * gj.x - zi.x * gj.y);
+              cr[u][v] += tr;
+              ci[u][v] += G3 ? ti + tr : ti;
             }
           }
         }
@@ -386,7 +395,16 @@ lindblad_rk4_resident(ResidentArgs a) {
           }
 #pragma unroll
           for (int u = 0; u < BPW && mstep; ++u) {
-            if constexpr (SPLIT) {
+            if constexpr (G3) {
+              const float sx = x[u].x + x[u].y, sy = y[u].x + y[u].y;
+              const float sz = z[u].x + z[u].y, dh = h[u].y - h[u].x;
+              cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
+              dr[u] = mfma_f32(x[u].y, y[u].y, dr[u]);
+              ci[u] = mfma_f32(sx, sy, ci[u]);
+              cr[u] = mfma_f32(-z[u].x, h[u].x, cr[u]);
+              dr[u] = mfma_f32(z[u].y, h[u].y, dr[u]);
+              ci[u] = mfma_f32(sz, dh, ci[u]);
+            } else if constexpr (SPLIT) {
               // Re C = Hx Sx - (Hy Sy + Sx Hx' + Sy Hy'),  Im C = Hx Sy + Hy Sx + Sx Hy' - Sy Hx'
               // (H' = H[j][k] of the S H^dag product)
               cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
@@ -439,7 +457,13 @@ lindblad_rk4_resident(ResidentArgs a) {
             h[u] = hn[u];
           }
         }
-        if constexpr (SPLIT) {
+        if constexpr (G3) {
+#pragma unroll
+          for (int u = 0; u < BPW; ++u) {
+            ci[u] -= cr[u] + dr[u];
+            cr[u] -= dr[u];
+          }
+        } else if constexpr (SPLIT) {
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
             cr[u] -= dr[u];
@@ -540,7 +564,7 @@ lindblad_rk4_resident(ResidentArgs a) {
             if constexpr (DFOLD) {
               const float2 p = tval[u] ? cmulf(dc[u][v], sv) : make_float2(0.f, 0.f);
               cr[u][v] = -p.y;
-              ci[u][v] = p.x;
+              ci[u][v] = G3 ? p.x - p.y : p.x;
             }
           }
         }
@@ -608,7 +632,7 @@ lindblad_rk4_resident(ResidentArgs a) {
           if constexpr (DFOLD) {
             const float2 p = tval[u] ? cmulf(dc[u][v], sv) : make_float2(0.f, 0.f);
             cr[u][v] = -p.y;
-            ci[u][v] = p.x;
+            ci[u][v] = G3 ? p.x - p.y : p.x;
           }
         }
       }
