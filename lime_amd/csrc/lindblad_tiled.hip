@@ -114,13 +114,20 @@ __device__ __forceinline__ void store_cols(C* sM, const C (&reg)[TILE * PW / 256
 template <int TILE>
 constexpr int smem_b() { return KW * (TILE + 1) > TILE * KWP ? KW * (TILE + 1) : TILE * KWP; }
 
-// Accumulate one product term into the wave's MFMA blocks.
+// Accumulate one product term into the wave's MFMA blocks, in the three-multiplication
+// (Gauss) form: three real products per complex product into
+//   cr = R1, dr = R2, ci = M1, with Re C = R1 - R2 and Im C = M1 - R1 - R2
+// (the conversion runs once, after the last term); per mode, with the stored B element h:
+//   PLAIN  (C += a h)         R1 += ax hx,   R2 += ay hy,  M1 += (ax + ay)(hx + hy)
+//   NEG_AH (C -= a conj(h))   R1 += -ax hx,  R2 += ay hy,  M1 += (ax + ay)(hy - hx)
+//   I_AH   (C += i a conj(h)) R1 += ax hy,   R2 += ay hx,  M1 += (ax + ay)(hx + hy)
 template <int TILE, int MODE, typename R>
 __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ A,
                                           const typename CT<R>::C* __restrict__ Bsrc, int n, int row0,
                                           int col0, typename CT<R>::C* sA, typename CT<R>::C* sB,
                                           typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
-                                          typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32], int tid, int wr,
+                                          typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
+                                          typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32], int tid, int wr,
                                           int wc, int lr, int lg) {
   using C = typename CT<R>::C;
   constexpr int BW = TILE / 32;
@@ -164,21 +171,19 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
         for (int q = 0; q < BW; ++q) {
           if (!(rlive[p] && clive[q])) continue;
           const C a = x[p], h = y[q];
+          const R as = a.x + a.y;
           if (MODE == PLAIN) {
             cr[p][q] = CT<R>::mfma(a.x, h.x, cr[p][q]);
-            ci[p][q] = CT<R>::mfma(a.x, h.y, ci[p][q]);
-            cr[p][q] = CT<R>::mfma(-a.y, h.y, cr[p][q]);
-            ci[p][q] = CT<R>::mfma(a.y, h.x, ci[p][q]);
-          } else if (MODE == NEG_AH) {   // C -= a conj(h)
+            dr[p][q] = CT<R>::mfma(a.y, h.y, dr[p][q]);
+            ci[p][q] = CT<R>::mfma(as, h.x + h.y, ci[p][q]);
+          } else if (MODE == NEG_AH) {
             cr[p][q] = CT<R>::mfma(-a.x, h.x, cr[p][q]);
-            ci[p][q] = CT<R>::mfma(a.x, h.y, ci[p][q]);
-            cr[p][q] = CT<R>::mfma(-a.y, h.y, cr[p][q]);
-            ci[p][q] = CT<R>::mfma(-a.y, h.x, ci[p][q]);
-          } else {                       // C += i a conj(h)
+            dr[p][q] = CT<R>::mfma(a.y, h.y, dr[p][q]);
+            ci[p][q] = CT<R>::mfma(as, h.y - h.x, ci[p][q]);
+          } else {
             cr[p][q] = CT<R>::mfma(a.x, h.y, cr[p][q]);
-            ci[p][q] = CT<R>::mfma(a.x, h.x, ci[p][q]);
-            cr[p][q] = CT<R>::mfma(-a.y, h.x, cr[p][q]);
-            ci[p][q] = CT<R>::mfma(a.y, h.y, ci[p][q]);
+            dr[p][q] = CT<R>::mfma(a.y, h.x, dr[p][q]);
+            ci[p][q] = CT<R>::mfma(as, h.x + h.y, ci[p][q]);
           }
         }
     }
@@ -314,14 +319,32 @@ template <int TILE, int MODE, typename R, int P>
 __device__ __forceinline__ void term(const typename CT<R>::C* __restrict__ A, const typename CT<R>::C* __restrict__ B,
                                      int n, int row0, int col0, unsigned char* smem,
                                      typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
-                                     typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32], int tid, int wr, int wc,
+                                     typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
+                                     typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
                                      int lr, int lg) {
   if constexpr (P == 1) {
     gemm_term_bf3<TILE, MODE>(A, B, n, row0, col0, smem, cr, ci, tid, wr, wc, lr, lg);
   } else {
     using C = typename CT<R>::C;
     C* sA = reinterpret_cast<C*>(smem);
-    gemm_term<TILE, MODE, R>(A, B, n, row0, col0, sA, sA + TILE * KWP, cr, ci, tid, wr, wc, lr, lg);
+    gemm_term<TILE, MODE, R>(A, B, n, row0, col0, sA, sA + TILE * KWP, cr, ci, dr, tid, wr, wc, lr, lg);
+  }
+}
+
+// (R1, R2, M1) of the FP32 / FP64 panels -> (Re C, Im C) in cr, ci; the 3xBF16 panels
+// accumulate Re C, Im C directly.
+template <int TILE, typename R, int P>
+__device__ __forceinline__ void finish_terms(typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
+                                             typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
+                                             const typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32]) {
+  if constexpr (P == 0) {
+#pragma unroll
+    for (int p = 0; p < TILE / 32; ++p)
+#pragma unroll
+      for (int q = 0; q < TILE / 32; ++q) {
+        ci[p][q] -= cr[p][q] + dr[p][q];
+        cr[p][q] -= dr[p][q];
+      }
   }
 }
 
@@ -373,13 +396,20 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
   const size_t nn = (size_t)n * n;
   const C* H = static_cast<const C*>(a.heff) + (size_t)set * nn;
   const C* S = static_cast<const C*>(a.s_cur) + (size_t)b * nn;
-  term<TILE, PLAIN, R, P>(H, S, n, g.row0, g.col0, smem, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
-  term<TILE, NEG_AH, R, P>(S, H, n, g.row0, g.col0, smem, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+  using V4 = typename CT<R>::V4;
+  V4 dr[TILE / 32][TILE / 32];
+#pragma unroll
+  for (int p = 0; p < TILE / 32; ++p)
+#pragma unroll
+    for (int q = 0; q < TILE / 32; ++q) dr[p][q] = V4{0, 0, 0, 0};
+  term<TILE, PLAIN, R, P>(H, S, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+  term<TILE, NEG_AH, R, P>(S, H, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
   for (int k = 0; k < a.n_dense; ++k) {
     const C* T = static_cast<const C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
     const C* Rk = static_cast<const C*>(a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + k) * nn;
-    term<TILE, I_AH, R, P>(T, Rk, n, g.row0, g.col0, smem, cr, ci, g.tid, g.wr, g.wc, g.lr, g.lg);
+    term<TILE, I_AH, R, P>(T, Rk, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
   }
+  finish_terms<TILE, R, P>(cr, ci, dr);
 }
 
 }  // namespace
@@ -398,14 +428,15 @@ __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
   const int set = b / a.group;
   const size_t nn = (size_t)a.n * a.n;
   constexpr int BW = TILE / 32;
-  V4 cr[BW][BW], ci[BW][BW];
+  V4 cr[BW][BW], ci[BW][BW], dr[BW][BW];
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
-    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
+    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = dr[p][q] = V4{0, 0, 0, 0};
   term<TILE, PLAIN, R, P>(static_cast<const C*>(a.lops) + ((size_t)set * a.n_dense + k) * nn,
                           static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, g.row0, g.col0, smem, cr, ci,
-                          g.tid, g.wr, g.wc, g.lr, g.lg);
+                          dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+  finish_terms<TILE, R, P>(cr, ci, dr);
   C* T = static_cast<C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
 #pragma unroll
   for (int p = 0; p < BW; ++p)
