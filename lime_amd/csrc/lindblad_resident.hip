@@ -132,11 +132,11 @@ lindblad_rk4_resident(ResidentArgs a) {
   // (512 / MINW VGPRs) covers ~56 VGPRs per tile + 72 of fixed state.
   constexpr bool SPLIT = 512 / MINW >= 56 * BPW + 72;
   // G3 (with SPLIT, no dense operators): the two complex products take 3 real products each
-  // (Gauss), 6 MFMAs per k-step instead of 8, into three accumulators
+  // (Gauss), 6 MFMAs per k-step instead of 8:
   //   R1 = Hx Sx - Sx Hx',  R2 = Hy Sy + Sy Hy',  M1 = (Hx + Hy)(Sx + Sy) - (Sx + Sy)(Hx' - Hy')
-  // (H' = H[j][k] of the S H^dag product), held in cr, dr, ci:
-  //   Re C = R1 - R2,  Im C = M1 - R1 - R2.
-  // A value (cr0, ci0) of C enters them as R1 = cr0, R2 = 0, M1 = ci0 + cr0.
+  // (H' = H[j][k] of the S H^dag product), Re C = R1 - R2, Im C = M1 - R1 - R2.  R1 is
+  // held as cr - di (Hx Sx in cr, Sx Hx' in di: no negated operand in the MFMA stream), R2
+  // in dr, M1 in ci.  A value (cr0, ci0) of C enters them as cr = cr0, ci = ci0 + cr0.
   constexpr bool G3 = SPLIT && !DENSE;
   // DB: double-buffered stage operand (when it fits the LDS; never with dense operators):
   // the epilogue writes the next S into the other buffer, so a stage needs one barrier
@@ -349,6 +349,7 @@ lindblad_rk4_resident(ResidentArgs a) {
           h[u] = sH[oHc[u]];
         }
         // SPLIT: cr, ci collect the positive products of Re C, Im C; dr, di the negative
+        // (G3: the four accumulators of the three-product form, see above)
         floatx4 dr[SPLIT ? BPW : 1], di[SPLIT ? BPW : 1];
 #pragma unroll
         for (int u = 0; u < (SPLIT ? BPW : 1); ++u) {
@@ -401,7 +402,7 @@ lindblad_rk4_resident(ResidentArgs a) {
               cr[u] = mfma_f32(x[u].x, y[u].x, cr[u]);
               dr[u] = mfma_f32(x[u].y, y[u].y, dr[u]);
               ci[u] = mfma_f32(sx, sy, ci[u]);
-              cr[u] = mfma_f32(-z[u].x, h[u].x, cr[u]);
+              di[u] = mfma_f32(z[u].x, h[u].x, di[u]);
               dr[u] = mfma_f32(z[u].y, h[u].y, dr[u]);
               ci[u] = mfma_f32(sz, dh, ci[u]);
             } else if constexpr (SPLIT) {
@@ -460,6 +461,7 @@ lindblad_rk4_resident(ResidentArgs a) {
         if constexpr (G3) {
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
+            cr[u] -= di[u];                   // R1
             ci[u] -= cr[u] + dr[u];
             cr[u] -= dr[u];
           }
