@@ -115,8 +115,8 @@ lindblad_rk4_resident(ResidentArgs a) {
   constexpr int NT = TG * TG;
   static_assert(W * BPW >= NT, "every tile needs an owner");
   constexpr int NC = 16 * (NB - 1);          // bordered: core size = border index
-  // bordered: BG lanes per border output, 2 NC groups of them (the corner rides with
-  // group 0); the groups fill whole waves (BG NC is a multiple of 32)
+  // bordered: BG lanes per border output, 2 NC groups of them (the corner is summed
+  // apart, below); the groups fill whole waves (BG NC is a multiple of 32)
   constexpr int BG = (T >= 8 * NC) ? 4 : 2;
   static_assert(!BRD || T >= 4 * NC, "every border output needs an owner");
   constexpr int BWAVES = BRD ? (2 * NC * BG) / 64 : 0;   // waves holding border groups
@@ -196,9 +196,10 @@ lindblad_rk4_resident(ResidentArgs a) {
   // ---- bordered: lane group g = tid / BG (g < 2 NC) sums border output (gi, gj), its lane
   // bq taking k = bq, bq + BG, ...; groups 0..NC-1 are the row outputs (NC, g), groups
   // NC..2NC-1 the column outputs (g - NC, NC).  The group's lane 0 owns its output's FP64
-  // state (bi, bj) = (gi, gj).  The corner (NC, NC) is summed by the first wave without
-  // border groups (CW = BWAVES, which has slack at the stage barrier; its lane 0 owns it),
-  // or, when every wave holds groups, by the groups of wave 0 (lane 1 owns it).
+  // state (bi, bj) = (gi, gj).  The corner (NC, NC) is summed after the k-loop, one k per
+  // lane and a wave sum, by the first wave without border groups (CW = BWAVES, which has
+  // slack at the stage barrier; its lane 0 owns it), or, when every wave holds groups, by
+  // wave 0 (lane 1 owns it).
   int gi = 0, gj = 0, bq = 0;
   bool bown = false;
   if (BRD) {
@@ -324,7 +325,7 @@ lindblad_rk4_resident(ResidentArgs a) {
       }
       br = 0.f;
       bm = 0.f;
-      float qr = 0.f, qm = 0.f;   // bordered: the corner's partial sum (wave 0)
+      float qr = 0.f, qm = 0.f;   // bordered: the corner's sum (wave CW)
       // C += H_eff S  and  C -= S H_eff^dag  ((S H^dag)_ij = sum_k S_ik conj(H_jk)).
       // Software pipelined by hand: fragments of step k0+4 are read from LDS while the
       // MFMAs of step k0 issue, so at most two fragment sets are live.  Bordered: the
@@ -440,15 +441,6 @@ lindblad_rk4_resident(ResidentArgs a) {
                 bm += hh.x * ss.y + hh.y * ss.x - (zz.y * gg.x - zz.x * gg.y);
               }
             }
-            if (w == CW) {   // the corner, summed by every lane group of wave CW alike
-#pragma unroll
-              for (int j = 0; j < 4 / BG; ++j) {
-                const int k = k0 + bq + BG * j;
-                const float2 hh = sH[NC * P + k], ss = sS[k * P + NC], zz = sS[NC * P + k];
-                qr += hh.x * ss.x - hh.y * ss.y - (zz.x * hh.x + zz.y * hh.y);
-                qm += hh.x * ss.y + hh.y * ss.x - (zz.y * hh.x - zz.x * hh.y);
-              }
-            }
           }
 #pragma unroll
           for (int u = 0; u < BPW; ++u) {
@@ -479,8 +471,18 @@ lindblad_rk4_resident(ResidentArgs a) {
         br = group_sum<BG>(br);
         bm = group_sum<BG>(bm);
         if (w == CW) {
-          qr = group_sum<BG>(qr);
-          qm = group_sum<BG>(qm);
+          // the corner, after the k-loop: lane l takes k = l (and l + 64), then a wave sum
+          // (inside the loop every lane group of the wave would repeat the same 4 terms)
+          for (int k = lane; k <= NC; k += 64) {
+            const float2 hh = sH[NC * P + k], ss = sS[k * P + NC], zz = sS[NC * P + k];
+            qr += hh.x * ss.x - hh.y * ss.y - (zz.x * hh.x + zz.y * hh.y);
+            qm += hh.x * ss.y + hh.y * ss.x - (zz.y * hh.x - zz.x * hh.y);
+          }
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) {
+            qr += __shfl_xor(qr, off, 64);
+            qm += __shfl_xor(qm, off, 64);
+          }
           if (bcorner) {
             br = qr;
             bm = qm;
