@@ -152,7 +152,7 @@ lindblad_rk4_resident(ResidentArgs a) {
   float2* sL = sT + (DENSE ? NP * P + 4 * P + 4 : 0);
   double2* sRed = reinterpret_cast<double2*>(sL + (DENSE ? NP * P + 4 * P + 4 : 0));
 
-  const int b = blockIdx.x;
+  const int b = xcd_block();   // the operators of one generator set run on one XCD (shared L2)
   if (b >= a.batch) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;

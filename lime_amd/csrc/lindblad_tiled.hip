@@ -350,21 +350,6 @@ __device__ __forceinline__ void finish_terms(typename CT<R>::V4 (&cr)[TILE / 32]
 
 __device__ __forceinline__ int tiles_per_dim(int n, int tile) { return (n + tile - 1) / tile; }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx b runs on XCD b % 8),
-// each with its own L2.  Renumber them so that XCD x runs a contiguous range of logical
-// ids: the tiles of one operator (consecutive logical ids) then share one L2 instead of
-// every XCD fetching the operator's panels.  A bijection for any grid size.
-__device__ __forceinline__ int xcd_block() {
-#ifdef LIME_NO_XCD_REMAP
-  return blockIdx.x;
-#else
-  constexpr int X = 8;
-  const int g = gridDim.x, b = blockIdx.x;
-  const int per = g / X, rem = g % X, x = b % X;
-  return x * per + (x < rem ? x : rem) + b / X;
-#endif
-}
-
 // Common per-workgroup geometry.
 struct Geo {
   int row0, col0, tid, lane, w, wr, wc, lr, lg;
