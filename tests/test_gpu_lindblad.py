@@ -235,6 +235,29 @@ def test_fp64_tiled_variant_matches_oracle(n, kd):
     assert np.max(np.abs(obs[0].cpu().numpy() - ref_obs)) < 1e-12
 
 
+@pytest.mark.parametrize("n", [17, 20])
+def test_batch_not_a_multiple_of_the_xcd_count(n):
+    """13 distinct operators (13 = 8 + 5: the workgroup renumbering over the 8 XCDs has a
+    full range and a remainder) on the bordered (n = 17) and padded (n = 20) resident
+    kernels: every operator lands on its own state and observables, checked against the
+    oracle one by one."""
+    rng = np.random.default_rng(1300 + n)
+    B = 13
+    Hs = np.stack([rand_herm(rng, n) for _ in range(B)])
+    c_ops = [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
+    X0 = np.stack([rand_c(rng, n, n) for _ in range(B)])
+    A = [rand_herm(rng, n)]
+    dt, nt = 0.02, 30
+    prop = _prop()(Hs, c_ops)
+    rho, obs, _ = prop.run(X0, dt, nt, e_ops=A, nout=5)
+    rho, obs = rho.cpu().numpy(), obs.cpu().numpy()
+    for b in range(B):
+        ref, ref_obs, _ = ol.propagate(Hs[b], X0[b], c_ops, dt, nt, A, nout=5)
+        scale = max(1.0, np.max(np.abs(ref)))
+        assert np.max(np.abs(rho[b] - ref)) / scale < 2e-7, b
+        assert np.max(np.abs(obs[b] - ref_obs)) < 1e-6 * max(1.0, np.max(np.abs(ref_obs))), b
+
+
 @pytest.mark.parametrize("n,kd", [(8, 1), (30, 2), (48, 0), (64, 2), (80, 0), (96, 0),
                                   (17, 0), (33, 0), (49, 0), (65, 0), (81, 0)])
 def test_every_launch_variant_matches_oracle(n, kd):
