@@ -13,6 +13,9 @@
   G = -i U.
 
 Pinned by: the Liouville form against the Hilbert form (independent constructions);
+the unitary limit (silent bath) of ``propagate`` against site-basis Hilbert-space RK4 and
+e^{-iHt} rho0 e^{iHt} at a non-diagonal H, which fixes the direction of the eigenbasis
+change on the way in and out;
 the population block of the secular Redfield rate matrix against golden-rule rates
 2|A_nm|^2 S(w) (closed form); trace conservation; exp(R t) against the eigendecomposition
 V e^{Lambda t} V^-1 and the semigroup law; the four-point response against the closed form

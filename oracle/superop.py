@@ -12,7 +12,8 @@ Row-major vectorisation ``vec(rho) = rho.flatten()`` so that ``vec(A rho B) = (A
   norms (lime keeps only the real part of diag(vl^dag vr), ``:509``).
 
 Pinned by: textbook identities vec(A X B) = (A (x) B^T) vec X checked on random
-matrices; scipy.linalg.expm of the Liouvillian; the closed form of dephasing.
+matrices; scipy.linalg.expm of the Liouvillian; the closed form of dephasing; ``trace`` and
+``obs`` against np.trace(rho) and np.trace(a rho) on complex non-Hermitian rho and a.
 """
 import numpy as np
 import scipy.linalg as sla

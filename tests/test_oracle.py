@@ -276,6 +276,47 @@ def test_redfield_rk4_trace_and_backtransform():
     assert np.all(pops > -1e-9)
 
 
+def test_redfield_unitary_limit_in_the_site_basis():
+    """With a silent bath (S = 0) the Redfield tensor is the bare commutator in the H
+    eigenbasis, so ``propagate`` (eigenbasis RK4, states transformed back) must reproduce
+    site-basis dynamics built without any basis change: the Hilbert-space RK4 of -i[H, rho]
+    (pinned by the cavity golden vectors and the diagonal closed form) step for step, and
+    e^{-iHt} rho0 e^{iHt} to the RK4 truncation error.  Observables are taken with
+    non-diagonal site operators.  Reversing the basis change (V rho V^dag on the way in,
+    V^dag rho V on the way out) fails every check here at a non-diagonal H."""
+    rng = np.random.default_rng(13)
+    n = 5
+    H = rand_herm(rng, n)
+    a_ops = [rand_herm(rng, n)]
+    R, evecs = orf.redfield_tensor(H, a_ops, [lambda w: 0.0])
+    assert np.max(np.abs(evecs - np.eye(n))) > 0.1          # genuinely non-diagonal H
+    rho0 = rand_dm(rng, n)
+    e_ops = [rand_herm(rng, n), rand_c(rng, n, n)]
+    dt, nt = 0.01, 150
+    states, obs = orf.propagate(R, evecs, rho0, dt, nt, e_ops)
+    _, obs_site, st_site = ol.propagate(H, rho0, [], dt, nt, e_ops=e_ops, store_states=True)
+    assert np.max(np.abs(states - np.array(st_site))) < 1e-12
+    assert np.max(np.abs(obs - obs_site)) < 1e-12
+    for k in (0, 49, nt - 1):
+        U = sla.expm(-1j * H * dt * (k + 1))
+        exact = U @ rho0 @ U.conj().T
+        assert np.max(np.abs(states[k] - exact)) < 1e-7     # RK4 truncation: 1.6e-8 here
+
+
+def test_superop_trace_and_obs_on_non_hermitian_vectors():
+    """``trace`` = Tr rho and ``obs`` = Tr[a rho] (lime superoperator.py:314-319) on a
+    complex, non-Hermitian rho and a: a conjugated rho (vdot argument order), a transposed
+    or conjugated a, or a column-major vec fails one of these."""
+    rng = np.random.default_rng(14)
+    n = 4
+    rho = rand_c(rng, n, n)
+    a = rand_c(rng, n, n)
+    v = osu.dm2vec(rho)
+    assert abs(osu.trace(v) - np.trace(rho)) < 1e-13
+    assert abs(osu.obs(v, a) - np.trace(a @ rho)) < 1e-12
+    assert abs(osu.obs(v, a) - np.trace(a.T @ rho)) > 1e-3       # the test can tell a from a^T
+
+
 # ---------------------------------------------------------------- FFT convention
 def test_fft_against_geometric_series():
     n, dt, x0 = 256, 0.1, 0.3
