@@ -97,7 +97,9 @@ typedef struct lime_step_params {
  * Fused batched Lindblad RK4: all four stages of `nsteps` steps in one launch, the
  * operator resident on chip, GEMMs on MFMA, state update / trace / observables in FP64.
  * Records Tr[E rho] = sum_ij E^T_ij rho_ij after global step g when (g+1) % nout == 0,
- * at record index (g+1)/nout - 1.
+ * at record index (g+1)/nout - 1.  Records are STORED (obs and states need no
+ * initialisation; entries of records outside this call are left untouched) and are
+ * bitwise reproducible from run to run on every path.
  *
  * Replaces the time loop of lime/oqs.py:1590-1688 (_lindblad: rk4 at phys.py:636-649
  * over liouvillian/lindbladian at oqs.py:706-723, obs_dm at phys.py:837-844) and of the

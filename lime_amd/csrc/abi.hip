@@ -75,7 +75,7 @@ int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
   if (use_resident(d)) return 0;
   return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
-                                             (d->flags & LIME_F_RENORM_TRACE) != 0);
+                                             (d->flags & LIME_F_RENORM_TRACE) != 0, d->obs ? d->n_eops : 0);
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
@@ -129,7 +129,7 @@ int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, v
 
   // ---- tiled path: one launch per RK4 stage, workspace supplied by the caller
   const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
-                                                 (d->flags & LIME_F_RENORM_TRACE) != 0);
+                                                 (d->flags & LIME_F_RENORM_TRACE) != 0, d->obs ? d->n_eops : 0);
   if (d->work == nullptr || d->work_bytes < (int64_t)need)
     return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n = " + std::to_string(d->n) + " needs a workspace of " +
                                   std::to_string(need) + " bytes (lime_lindblad_workspace_bytes)");

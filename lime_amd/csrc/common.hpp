@@ -62,14 +62,10 @@ __host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 // sharing one generator) then share one L2 instead of every XCD fetching the same data.
 // A bijection for any grid size.
 __device__ __forceinline__ int xcd_block() {
-#ifdef LIME_NO_XCD_REMAP
-  return blockIdx.x;
-#else
   constexpr int X = 8;
   const int g = gridDim.x, b = blockIdx.x;
   const int per = g / X, rem = g % X, x = b % X;
   return x * per + (x < rem ? x : rem) + b / X;
-#endif
 }
 
 }  // namespace lime

@@ -10,8 +10,10 @@
 //   * 4 waves in a 2 x 2 arrangement, each owning (TILE/32)^2 16x16 MFMA blocks;
 //   * the RK4 stage update runs in the epilogue: k = -i C + D o S, the increment, the next
 //     stage operand (double buffered so no tile reads a half-written operand), and at the
-//     last stage rho += increment in FP64, records (observables by FP64 atomics, states)
-//     and the next S = rho.
+//     last stage rho += increment in FP64, records (each workgroup's share of Tr[E rho] to
+//     its own workspace slot, summed over the slots in a fixed order by obs_reduce, so the
+//     records are stored, never accumulated, and bitwise reproducible; states) and the
+//     next S = rho.
 // Three arithmetic variants share the code (their MFMA accumulator row layouts differ, see
 // CT::row):
 //   R = float,  P = 0: complex64 operands on v_mfma_f32_16x16x4_f32, FP32 increment;
@@ -29,10 +31,7 @@ namespace lime {
 namespace {
 
 constexpr int KP = 16;          // K-panel width (complex elements) of the 3xBF16 staging
-#ifndef LIME_TILED_KW
-#define LIME_TILED_KW 16
-#endif
-constexpr int KW = LIME_TILED_KW;   // K-panel width of the FP32 / FP64 staging
+constexpr int KW = 16;          // K-panel width of the FP32 / FP64 staging
 constexpr int KWP = KW + 1;         // LDS pitch of [row][k] panels (odd: conflict-free fragments)
 
 template <typename R> struct CT;
@@ -518,10 +517,10 @@ __device__ __forceinline__ void apply_elem(const TiledArgs& a, int b, int set, s
   }
 }
 
-// Adds the workgroup's share (pr, pi: this lane's part) of Tr[E_e rho] at the record of
-// this step to the observables with FP64 atomics.  Called by every thread of the group.
-__device__ __forceinline__ void obs_atomic(const TiledArgs& a, int b, double pr, double pi, int e,
-                                           double2* sRed) {
+// Stores the workgroup's share (pr, pi: this lane's part) of Tr[E_e rho] at this step in
+// the workgroup's slot of obs_part; obs_reduce sums the slots.  Called by every thread.
+__device__ __forceinline__ void obs_partial(const TiledArgs& a, int b, int slot, double pr, double pi, int e,
+                                            double2* sRed) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const double sr = wave_sum(pr), si = wave_sum(pi);
   __syncthreads();
@@ -533,9 +532,7 @@ __device__ __forceinline__ void obs_atomic(const TiledArgs& a, int b, double pr,
       t.x += sRed[x].x;
       t.y += sRed[x].y;
     }
-    double* dst = reinterpret_cast<double*>(a.obs + ((size_t)b * a.rec_capacity + a.rec) * a.n_eops + e);
-    atomicAdd(dst, t.x);
-    atomicAdd(dst + 1, t.y);
+    a.obs_part[((size_t)b * a.slots + slot) * a.n_eops + e] = t;
   }
 }
 
@@ -620,7 +617,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
           pr = ev.x * rf.x - ev.y * rf.y;
           pi = ev.x * rf.y + ev.y * rf.x;
         }
-        obs_atomic(a, b, pr, pi, e, sRed);
+        obs_partial(a, b, idx, pr, pi, e, sRed);
       }
     }
     return;
@@ -666,7 +663,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
             pr += ev.x * rfin[p][q][v].x - ev.y * rfin[p][q][v].y;
             pi += ev.x * rfin[p][q][v].y + ev.y * rfin[p][q][v].x;
           }
-      obs_atomic(a, b, pr, pi, e, sRed);
+      obs_partial(a, b, idx, pr, pi, e, sRed);
     }
   }
 }
@@ -868,8 +865,24 @@ static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) 
   return hipGetLastError();
 }
 
+// Record of the step's observables: the sum of the per-workgroup partials of every
+// operator, in slot order (deterministic), stored at record index a.rec.
+__global__ void __launch_bounds__(64) obs_reduce(TiledArgs a) {
+  const int b = blockIdx.x;
+  for (int e = threadIdx.x; e < a.n_eops; e += 64) {
+    double2 t = make_double2(0.0, 0.0);
+    const double2* p = a.obs_part + (size_t)b * a.slots * a.n_eops + e;
+    for (int x = 0; x < a.slots; ++x) {
+      const double2 y = p[(size_t)x * a.n_eops];
+      t.x += y.x;
+      t.y += y.y;
+    }
+    a.obs[((size_t)b * a.rec_capacity + a.rec) * a.n_eops + e] = t;
+  }
+}
+
 // Trace renormalisation after a step (flag bit 0; density matrices only): the stage-3
-// kernels have written rho and accumulated the step's observables.  renorm_trace computes
+// kernels have written rho and obs_reduce the step's observables.  renorm_trace computes
 // s_b = 1 / Re Tr rho_b in FP64 (one workgroup per operator) and rescales the observables
 // recorded at this step; renorm_scale applies s_b to rho, the next stage operand and the
 // recorded state.  Observables and states are therefore recorded after renormalisation.
@@ -922,11 +935,40 @@ __global__ void init_stage_operand(const double2* __restrict__ rho, C* __restric
   }
 }
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm) {
-  const size_t nn = (size_t)n * n;
-  const size_t elem = fp64 ? sizeof(double2) : sizeof(float2);
-  const size_t ops = (size_t)batch * nn * elem * (3 + (size_t)n_dense);   // S0, S1, inc, T_k
-  return renorm ? ops + (size_t)batch * sizeof(double) : ops;             // + trace scales
+// Byte offsets of the workspace regions (see tiled_workspace_bytes).
+struct WorkLayout {
+  size_t s0, s1, inc, tbuf, tscale, obs_part, total;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workgroups per operator of the stage kernel for a tile edge (the bordered layout adds
+// the border workgroups).
+static int stage_wgs(int n, int n_dense, int tile) {
+  TiledArgs t = {};
+  t.n = n;
+  t.n_dense = n_dense;
+  return layout(t, tile);
+}
+
+static WorkLayout work_layout(int batch, int n, int n_dense, bool fp64, bool renorm, int n_eops) {
+  const size_t op = (size_t)batch * n * n * (fp64 ? sizeof(double2) : sizeof(float2));
+  WorkLayout w;
+  w.s0 = 0;
+  w.s1 = align256(op);
+  w.inc = w.s1 + align256(op);
+  w.tbuf = w.inc + align256(op);
+  w.tscale = w.tbuf + align256(op * (size_t)n_dense);
+  w.obs_part = w.tscale + (renorm ? align256((size_t)batch * sizeof(double)) : 0);
+  // slots for the larger of the two tile choices (32-tiles: more workgroups)
+  const int slots = stage_wgs(n, n_dense, 32) > stage_wgs(n, n_dense, 64) ? stage_wgs(n, n_dense, 32)
+                                                                          : stage_wgs(n, n_dense, 64);
+  w.total = w.obs_part + (n_eops > 0 ? align256((size_t)batch * slots * n_eops * sizeof(double2)) : 0);
+  return w;
+}
+
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm, int n_eops) {
+  return work_layout(batch, n, n_dense, fp64, renorm, n_eops).total;
 }
 
 int tiled_pick(int batch, int n) {
@@ -939,14 +981,17 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
                           int tile) {
   using C = typename CT<R>::C;
   const size_t nn = (size_t)a.n * a.n;
-  C* s0 = static_cast<C*>(work);
-  C* s1 = s0 + (size_t)a.batch * nn;
-  C* inc = s1 + (size_t)a.batch * nn;
-  a.inc = inc;
-  a.tbuf = inc + (size_t)a.batch * nn;
+  const WorkLayout wl = work_layout(a.batch, a.n, a.n_dense, sizeof(R) == 8, (a.flags & 1) != 0,
+                                    a.obs ? a.n_eops : 0);
+  unsigned char* base = static_cast<unsigned char*>(work);
+  C* s0 = reinterpret_cast<C*>(base + wl.s0);
+  C* s1 = reinterpret_cast<C*>(base + wl.s1);
+  a.inc = base + wl.inc;
+  a.tbuf = base + wl.tbuf;
+  a.tscale = reinterpret_cast<double*>(base + wl.tscale);
+  a.obs_part = reinterpret_cast<double2*>(base + wl.obs_part);
+  a.slots = stage_wgs(a.n, a.n_dense, tile);
   const size_t cnt = (size_t)a.batch * nn;
-  // trace scales follow the operand buffers (S0, S1, inc, T_k)
-  a.tscale = reinterpret_cast<double*>(static_cast<C*>(a.tbuf) + (size_t)a.batch * a.n_dense * nn);
   hipLaunchKernelGGL(init_stage_operand<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, a.rho,
                      s0, cnt);
   for (int s = 0; s < nsteps; ++s) {
@@ -957,6 +1002,7 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
     // stage 3 writes S = rho into s0 (stages alternate s0 -> s1 -> s0 -> s1 -> s0)
     hipError_t e = (tile == 64) ? run_step<64, R, P>(a, stream, s0, s1) : run_step<32, R, P>(a, stream, s0, s1);
     if (e != hipSuccess) return e;
+    if (a.rec >= 0 && a.obs) hipLaunchKernelGGL(obs_reduce, dim3(a.batch), dim3(64), 0, stream, a);
     if (a.flags & 1) {
       hipLaunchKernelGGL(renorm_trace, dim3(a.batch), dim3(256), 0, stream, a);
       const size_t blocks = (cnt + 255) / 256;

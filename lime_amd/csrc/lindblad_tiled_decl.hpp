@@ -26,6 +26,8 @@ struct TiledArgs {
   void* inc;
   void* tbuf;
   double* tscale;          // per-operator 1 / Re Tr rho of the trace renormalisation
+  double2* obs_part;       // [batch][slots][n_eops] per-workgroup partial sums of Tr[E rho]
+  int slots;               // workgroups per operator of the stage kernel (= obs_part slots)
   double2* stage_rec;      // [rec_capacity][3][batch][n][n] FP64 stage operands, or NULL
   // generator application (lindblad_apply_tiled): V = L(s_cur);
   //   y = c_lx V + c_z z (complex128), y64 = copy of y in the operand precision,
@@ -41,7 +43,10 @@ struct TiledArgs {
 // split products on BF16 MFMA, complex128 on FP64 MFMA.
 enum class TiledPrec { FP32 = 0, SPLIT_BF16 = 1, FP64 = 2 };
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm);
+// Workspace of lime_lindblad_rk4's per-stage path: S0, S1, the increment, T_k (operand
+// precision), the trace scales (renormalisation) and the per-workgroup observable partials
+// (n_eops > 0), each region 256-byte aligned.
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm, int n_eops);
 int tiled_pick(int batch, int n);
 hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
                      int tile, TiledPrec prec);

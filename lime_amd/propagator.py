@@ -192,14 +192,23 @@ class LindbladPropagator:
     def batch(self):
         return self.nsets * self.group
 
+    def _resident(self):
+        """Does ``step`` run the resident kernel (whose ``variant`` is a launch shape, not
+        the tile edge of the per-stage kernels)?"""
+        return not self.force_tiled and self.precision == "fp32" and self.n <= self.max_n()
+
+    def _tile_variant(self):
+        """The ``variant`` field for the per-stage kernels (``apply`` always runs them): a
+        tuned tile choice carries over, a resident launch shape does not."""
+        return 0 if self._resident() else self.variant
+
     def tiled(self):
         """The same operator batch on the per-stage kernels (shares the device operators)."""
         t = object.__new__(LindbladPropagator)
         t.__dict__.update(self.__dict__)
         t._work = None
-        resident = (not self.force_tiled and self.precision == "fp32" and self.n <= self.max_n())
+        t.variant = self._tile_variant()
         t.force_tiled = True
-        t.variant = 0 if resident else self.variant     # a tuned tile choice carries over
         return t
 
     def adjoint(self):
@@ -208,6 +217,8 @@ class LindbladPropagator:
         adj = object.__new__(LindbladPropagator)
         adj.__dict__.update(self.__dict__)
         adj._work = None
+        adj.variant = self._tile_variant()
+        adj.force_tiled = True
         h = lambda t: t.conj().transpose(-1, -2).contiguous()
         adj.heff = (-h(self.heff)).contiguous()
         adj.heff_c128 = (-h(self.heff_c128)).contiguous()
@@ -246,6 +257,7 @@ class LindbladPropagator:
         if y64 is not None:
             _check(y64, "y64 (operand precision)", (B, n, n), op_dtype, self.dev)
         d = self._descriptor(B)
+        d.variant = self._tile_variant()
         if self.n_dense > 0:
             need = B * self.n_dense * self.n * self.n * (16 if self.precision == "fp64" else 8)
             if self._work is None or self._work.numel() < need:

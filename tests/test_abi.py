@@ -85,15 +85,27 @@ def test_argument_validation_without_gpu(lib):
         _lib.check(_call(lib, n=200))
 
 
+def _a256(x):
+    return (x + 255) // 256 * 256
+
+
 def test_workspace_query(lib):
+    """Regions S0, S1, increment, T_k (operand precision), trace scales, observable
+    partials (one slot per stage workgroup of the 32-tile layout), each 256-byte aligned."""
     d = _lib.LindbladBatch()
     d.batch, d.n, d.group, d.eop_group = 4, 64, 1, 1
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 0          # resident
     d.flags = _lib.LIME_F_FORCE_TILED
-    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 64 * 64 * 8 * 3
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 64 * 64 * 8)
     d.flags, d.n, d.n_dense = 0, 130, 2
-    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 130 * 130 * 8 * 5
+    op = _a256(4 * 130 * 130 * 8)
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * op + _a256(2 * 4 * 130 * 130 * 8)
     d.flags, d.n, d.n_dense = _lib.LIME_F_PREC_3XBF16, 16, 0                 # complex64 workspace
-    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 16 * 16 * 8 * 3
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 16 * 16 * 8)
     d.flags = _lib.LIME_F_PREC_3XBF16 | _lib.LIME_F_RENORM_TRACE             # + trace scales
-    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 4 * 16 * 16 * 8 * 3 + 4 * 8
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 16 * 16 * 8) + 256
+    # observables: n = 129 (bordered 32-tiles: 16 tiles + 9 border workgroups per operator)
+    d.flags, d.n, d.n_dense, d.n_eops = _lib.LIME_F_PREC_FP64, 129, 0, 3
+    d.obs = 256                                                              # never dereferenced
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == \
+        3 * _a256(4 * 129 * 129 * 16) + _a256(4 * 25 * 3 * 16)
