@@ -108,6 +108,22 @@ typedef struct lime_step_params {
  */
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream);
 
+/* Size of the static device scratch array of lime_lindblad_rk4_noalloc (per GPU). */
+#define LIME_NOALLOC_SCRATCH_BYTES (64ll << 20)
+
+/*
+ * lime_lindblad_rk4 without any allocation and without a caller workspace: the per-stage
+ * kernels run on a static device array of LIME_NOALLOC_SCRATCH_BYTES per GPU (held by
+ * each GPU the library runs on until the process ends); d->work / d->work_bytes are
+ * ignored.  Calls no hipMalloc / hipFree.  Returns LIME_ERR_ARG when the batch needs more
+ * scratch than that (lime_lindblad_workspace_bytes tells).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (concurrent streams or threads) race.
+ * Nothing in the array needs to start at zero; it keeps the last call's contents.
+ * No counterpart in lime: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ */
+int lime_lindblad_rk4_noalloc(const lime_lindblad_batch* d, const lime_step_params* p, void* stream);
+
 /*
  * One application of the generator to every operator of the batch (adjoint and stage
  * recomputation building block):

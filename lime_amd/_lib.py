@@ -20,10 +20,12 @@ LIME_F_PREC_FP32 = 0x00
 LIME_F_PREC_3XBF16 = 0x10
 LIME_F_PREC_FP64 = 0x20
 LIME_F_FORCE_TILED = 0x100
+LIME_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 #: every symbol include/lime_hip.h declares (checked by tests/test_abi.py)
 EXPORTS = (
     "lime_lindblad_rk4",
+    "lime_lindblad_rk4_noalloc",
     "lime_lindblad_max_n",
     "lime_lindblad_workspace_bytes",
     "lime_lindblad_variants",
@@ -98,6 +100,9 @@ def load(path=LIB_PATH):
         lib.lime_lindblad_rk4.argtypes = [ctypes.POINTER(LindbladBatch),
                                           ctypes.POINTER(StepParams), ctypes.c_void_p]
         lib.lime_lindblad_rk4.restype = ctypes.c_int
+        lib.lime_lindblad_rk4_noalloc.argtypes = [ctypes.POINTER(LindbladBatch),
+                                                  ctypes.POINTER(StepParams), ctypes.c_void_p]
+        lib.lime_lindblad_rk4_noalloc.restype = ctypes.c_int
         lib.lime_lindblad_max_n.argtypes = [ctypes.c_int32, ctypes.c_int32]
         lib.lime_lindblad_max_n.restype = ctypes.c_int
         lib.lime_lindblad_workspace_bytes.argtypes = [ctypes.POINTER(LindbladBatch)]

@@ -1,5 +1,6 @@
 // extern "C" entry points of liblime_hip.so (declared in include/lime_hip.h).
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -50,6 +51,130 @@ int tiled_tile(const lime_lindblad_batch* d) {
   return lime::tiled_pick(d->batch, d->n);
 }
 
+// ---- static scratch of lime_lindblad_rk4_noalloc: one array per GPU, held until the
+// process ends.  Host code never uses the array's name as an address (on the host it is a
+// stand-in); the device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kScratchBytes = (size_t)LIME_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the first call on
+// that device, with that device current).
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4_noalloc: device index " + std::to_string(dev) + " >= 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4_noalloc: hipGetSymbolAddress(scratch)");
+    if (ptr == nullptr) return fail(LIME_ERR_HIP, "lime_lindblad_rk4_noalloc: scratch has no device address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return LIME_OK;
+}
+
+// lime_lindblad_rk4 with the per-stage workspace given explicitly (the caller's, or the
+// library's static scratch array for lime_lindblad_rk4_noalloc); `name` prefixes errors.
+int rk4_impl(const lime_lindblad_batch* d, const lime_step_params* p, void* stream, void* work,
+             int64_t work_bytes, const char* name, bool noalloc) {
+  if (d == nullptr || p == nullptr) return fail(LIME_ERR_ARG, std::string(name) + ": null descriptor");
+  if (d->batch < 0 || d->n < 1) return fail(LIME_ERR_ARG, std::string(name) + ": batch must be >= 0 and n >= 1");
+  if (d->batch == 0 || p->nsteps == 0) return LIME_OK;
+  if (p->nsteps < 0) return fail(LIME_ERR_ARG, std::string(name) + ": nsteps < 0");
+  if (d->group < 1 || d->eop_group < 1) return fail(LIME_ERR_ARG, std::string(name) + ": group and eop_group must be >= 1");
+  if (d->heff == nullptr || d->rho == nullptr) return fail(LIME_ERR_ARG, std::string(name) + ": heff and rho are required");
+  if (d->n_dense < 0 || (d->n_dense > 0 && d->lops == nullptr))
+    return fail(LIME_ERR_ARG, std::string(name) + ": n_dense > 0 needs lops");
+  if (d->n_eops < 0) return fail(LIME_ERR_ARG, std::string(name) + ": n_eops < 0");
+  if (d->obs != nullptr && (d->n_eops == 0 || d->eops_t == nullptr))
+    return fail(LIME_ERR_ARG, std::string(name) + ": obs needs n_eops > 0 and eops_t");
+  if ((d->obs != nullptr || d->states != nullptr || d->stage_states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))
+    return fail(LIME_ERR_ARG, std::string(name) + ": recording needs nout >= 1 and rec_capacity >= 0");
+  if (p->step_offset < 0) return fail(LIME_ERR_ARG, std::string(name) + ": step_offset < 0");
+  if ((d->flags & LIME_F_PREC_MASK) == 0x30)
+    return fail(LIME_ERR_UNSUPPORTED, std::string(name) + ": precision flag not built");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+
+  if (use_resident(d)) {
+    if (d->stage_states != nullptr)
+      return fail(LIME_ERR_UNSUPPORTED, std::string(name) + ": stage operands are recorded by the per-stage "
+                                        "kernels only (set LIME_F_FORCE_TILED)");
+    lime::ResidentArgs a;
+    a.batch = d->batch;
+    a.n = d->n;
+    a.group = d->group;
+    a.n_dense = d->n_dense;
+    a.n_eops = d->obs ? d->n_eops : 0;
+    a.eop_group = d->eop_group;
+    a.flags = d->flags;
+    a.heff = static_cast<const float2*>(d->heff);
+    a.dmat = static_cast<const float2*>(d->dmat);
+    a.lops = static_cast<const float2*>(d->lops);
+    a.rops = static_cast<const float2*>(d->rops);
+    a.eops_t = static_cast<const double2*>(d->eops_t);
+    a.rho = static_cast<double2*>(d->rho);
+    a.obs = static_cast<double2*>(d->obs);
+    a.states = static_cast<double2*>(d->states);
+    a.dt = p->dt;
+    a.nsteps = p->nsteps;
+    a.nout = (d->obs || d->states) ? p->nout : 0;
+    a.step_offset = p->step_offset;
+    a.rec_capacity = p->rec_capacity;
+    hipError_t e = lime::resident_launch(a, resident_variant(d), s);
+    if (e != hipSuccess) return hip_fail(e, (std::string(name) + " (resident) launch").c_str());
+    return LIME_OK;
+  }
+
+  // ---- tiled path: one launch per RK4 stage on the workspace
+  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
+                                                 (d->flags & LIME_F_RENORM_TRACE) != 0, d->obs ? d->n_eops : 0);
+  if (noalloc && need > kScratchBytes)
+    return fail(LIME_ERR_ARG, std::string(name) + ": this batch needs " + std::to_string(need) +
+                                  " bytes of scratch, above the static " + std::to_string(kScratchBytes) +
+                                  " (LIME_NOALLOC_SCRATCH_BYTES); use lime_lindblad_rk4 with a workspace");
+  if (noalloc) {
+    const int r = scratch_for_current_device(&work);
+    if (r != LIME_OK) return r;
+    work_bytes = (int64_t)kScratchBytes;
+  }
+  if (work == nullptr || work_bytes < (int64_t)need)
+    return fail(LIME_ERR_ARG, std::string(name) + ": n = " + std::to_string(d->n) + " needs a workspace of " +
+                                  std::to_string(need) + " bytes (lime_lindblad_workspace_bytes)");
+  lime::TiledArgs a = {};
+  a.batch = d->batch;
+  a.n = d->n;
+  a.group = d->group;
+  a.n_dense = d->n_dense;
+  a.n_eops = d->obs ? d->n_eops : 0;
+  a.eop_group = d->eop_group;
+  a.flags = d->flags;
+  a.rec_capacity = p->rec_capacity;
+  a.dt = p->dt;
+  a.heff = d->heff;
+  a.dmat = d->dmat;
+  a.lops = d->lops;
+  a.rops = d->rops;
+  a.eops_t = static_cast<const double2*>(d->eops_t);
+  a.rho = static_cast<double2*>(d->rho);
+  a.obs = static_cast<double2*>(d->obs);
+  a.states = static_cast<double2*>(d->states);
+  a.stage_rec = static_cast<double2*>(d->stage_states);
+  const int nout = (d->obs || d->states || d->stage_states) ? p->nout : 0;
+  const lime::TiledPrec prec = is_fp64(d) ? lime::TiledPrec::FP64
+                              : is_split_bf16(d) ? lime::TiledPrec::SPLIT_BF16 : lime::TiledPrec::FP32;
+  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, work, s, tiled_tile(d), prec);
+  if (e != hipSuccess) return hip_fail(e, (std::string(name) + " (tiled) launch").c_str());
+  return LIME_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -79,85 +204,11 @@ int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
-  if (d == nullptr || p == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: null descriptor");
-  if (d->batch < 0 || d->n < 1) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: batch must be >= 0 and n >= 1");
-  if (d->batch == 0 || p->nsteps == 0) return LIME_OK;
-  if (p->nsteps < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: nsteps < 0");
-  if (d->group < 1 || d->eop_group < 1) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: group and eop_group must be >= 1");
-  if (d->heff == nullptr || d->rho == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: heff and rho are required");
-  if (d->n_dense < 0 || (d->n_dense > 0 && d->lops == nullptr))
-    return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n_dense > 0 needs lops");
-  if (d->n_eops < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n_eops < 0");
-  if (d->obs != nullptr && (d->n_eops == 0 || d->eops_t == nullptr))
-    return fail(LIME_ERR_ARG, "lime_lindblad_rk4: obs needs n_eops > 0 and eops_t");
-  if ((d->obs != nullptr || d->states != nullptr || d->stage_states != nullptr) && (p->nout < 1 || p->rec_capacity < 0))
-    return fail(LIME_ERR_ARG, "lime_lindblad_rk4: recording needs nout >= 1 and rec_capacity >= 0");
-  if (p->step_offset < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4: step_offset < 0");
-  if ((d->flags & LIME_F_PREC_MASK) == 0x30)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: precision flag not built");
-  const hipStream_t s = static_cast<hipStream_t>(stream);
+  return rk4_impl(d, p, stream, d ? d->work : nullptr, d ? d->work_bytes : 0, "lime_lindblad_rk4", false);
+}
 
-  if (use_resident(d)) {
-    if (d->stage_states != nullptr)
-      return fail(LIME_ERR_UNSUPPORTED, "lime_lindblad_rk4: stage operands are recorded by the per-stage "
-                                        "kernels only (set LIME_F_FORCE_TILED)");
-    lime::ResidentArgs a;
-    a.batch = d->batch;
-    a.n = d->n;
-    a.group = d->group;
-    a.n_dense = d->n_dense;
-    a.n_eops = d->obs ? d->n_eops : 0;
-    a.eop_group = d->eop_group;
-    a.flags = d->flags;
-    a.heff = static_cast<const float2*>(d->heff);
-    a.dmat = static_cast<const float2*>(d->dmat);
-    a.lops = static_cast<const float2*>(d->lops);
-    a.rops = static_cast<const float2*>(d->rops);
-    a.eops_t = static_cast<const double2*>(d->eops_t);
-    a.rho = static_cast<double2*>(d->rho);
-    a.obs = static_cast<double2*>(d->obs);
-    a.states = static_cast<double2*>(d->states);
-    a.dt = p->dt;
-    a.nsteps = p->nsteps;
-    a.nout = (d->obs || d->states) ? p->nout : 0;
-    a.step_offset = p->step_offset;
-    a.rec_capacity = p->rec_capacity;
-    hipError_t e = lime::resident_launch(a, resident_variant(d), s);
-    if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (resident) launch");
-    return LIME_OK;
-  }
-
-  // ---- tiled path: one launch per RK4 stage, workspace supplied by the caller
-  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
-                                                 (d->flags & LIME_F_RENORM_TRACE) != 0, d->obs ? d->n_eops : 0);
-  if (d->work == nullptr || d->work_bytes < (int64_t)need)
-    return fail(LIME_ERR_ARG, "lime_lindblad_rk4: n = " + std::to_string(d->n) + " needs a workspace of " +
-                                  std::to_string(need) + " bytes (lime_lindblad_workspace_bytes)");
-  lime::TiledArgs a = {};
-  a.batch = d->batch;
-  a.n = d->n;
-  a.group = d->group;
-  a.n_dense = d->n_dense;
-  a.n_eops = d->obs ? d->n_eops : 0;
-  a.eop_group = d->eop_group;
-  a.flags = d->flags;
-  a.rec_capacity = p->rec_capacity;
-  a.dt = p->dt;
-  a.heff = d->heff;
-  a.dmat = d->dmat;
-  a.lops = d->lops;
-  a.rops = d->rops;
-  a.eops_t = static_cast<const double2*>(d->eops_t);
-  a.rho = static_cast<double2*>(d->rho);
-  a.obs = static_cast<double2*>(d->obs);
-  a.states = static_cast<double2*>(d->states);
-  a.stage_rec = static_cast<double2*>(d->stage_states);
-  const int nout = (d->obs || d->states || d->stage_states) ? p->nout : 0;
-  const lime::TiledPrec prec = is_fp64(d) ? lime::TiledPrec::FP64
-                              : is_split_bf16(d) ? lime::TiledPrec::SPLIT_BF16 : lime::TiledPrec::FP32;
-  hipError_t e = lime::tiled_run(a, p->nsteps, p->step_offset, nout, d->work, s, tiled_tile(d), prec);
-  if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4 (tiled) launch");
-  return LIME_OK;
+int lime_lindblad_rk4_noalloc(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
+  return rk4_impl(d, p, stream, nullptr, 0, "lime_lindblad_rk4_noalloc", true);
 }
 
 int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const void* z, void* y,
