@@ -33,21 +33,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(args):
+RESIDENT_SOURCES = ("common.hpp", "lindblad_resident.hip", "lindblad_resident_decl.hpp")
+
+
+def resident_source_hash():
+    """First 16 hex digits of the SHA-256 of the resident kernel's source files."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lime_amd", "csrc")
+    for name in RESIDENT_SOURCES:
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profiled_traffic(args, variant):
     """HBM bytes per launch of the default workload from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel calibration), or None when the
-    run does not match the profiled configuration."""
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a copy-kernel calibration).  Returns
+    (bytes or None, note): None when the run's configuration, launch variant or resident
+    kernel source differ from the profiled ones (the figure would be stale)."""
     if (args.realisations, args.rk_per_step) != (4096, 256):
-        return None
+        return None, "not the profiled configuration (4096 realisations, 256 steps per launch)"
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "lindblad_rk4_resident_lattice65_traffic.txt")
+    fields = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None, f"{path} missing"
+    if fields.get("variant") != str(variant):
+        return None, f"stale: profiled variant {fields.get('variant')}, this run {variant}"
+    if fields.get("source_sha16") != resident_source_hash():
+        return None, "stale: the resident kernel source changed since it was profiled"
+    try:
+        return float(fields["traffic_bytes_per_launch"]), "profiles/lindblad_rk4_resident_lattice65_traffic.txt"
+    except (KeyError, ValueError):
+        return None, "no traffic_bytes_per_launch line"
 
 
 def flops_per_rho_step(n, k_dense=0):
@@ -100,15 +124,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU comparator")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy checks")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the dense-operand leg of the default workload")
     ap.add_argument("--workload", default="lattice65",
                     choices=["lattice65", "ragged", "gradient", "dense512"],
                     help="lattice65 (default): 4096 x 3 QRT operators at n=65; ragged: 2048 "
                          "instances n in {16..256} in one graph; gradient: adjoint of the "
                          "128-site chain spectrum loss; dense512: n=512 with 16 dense L_k")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "3xbf16", "fp64"],
-                    help="GEMM arithmetic (ragged / dense512)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "3xbf16", "fp64", "fp32/fp64"],
+                    help="GEMM arithmetic (ragged / dense512 / gradient; 'fp32/fp64' = FP32 forward "
+                         "with FP64 adjoint applications, gradient only)")
     ap.add_argument("--batch", type=int, default=32, help="dense512: operators per launch")
     ap.add_argument("--instances", type=int, default=2048, help="ragged: number of instances")
+    ap.add_argument("--ragged-mode", default="persistent", choices=["persistent", "classes"],
+                    help="ragged: one persistent launch (default) or the per-class graph")
     ap.add_argument("--check-steps", type=int, default=100,
                     help="dense512: RK4 steps of the FP64 GPU reference comparison")
     ap.add_argument("--variant", type=int, default=None,
@@ -195,18 +224,25 @@ def main():
     log(f"[bench] timed {args.steps} launches: {total_ms:.1f} ms (wall {wall:.2f} s), "
         f"{value:.4g} rho-steps/s, {achieved:.2f} TFLOP/s")
 
-    # spectrum of the isotropic average (the tail of the pipeline; not timed)
-    C = obs[:, :, 0].reshape(R, 3, total_steps).sum(dim=1) / 3.0
-    Cavg = C.mean(dim=0)
-    spec = torch.fft.fftshift(torch.fft.fft(Cavg)) * dt
+    # spectrum of the isotropic ensemble average over the workload's nt steps (the tail of
+    # the pipeline, lime's transform; not timed)
+    from lime_amd import fft as gfft
+    nspec = min(cfg["nt"], total_steps)
+    C = obs[:, :nspec, 0].reshape(R, 3, nspec).sum(dim=1) / 3.0
+    spec, _ = gfft.fft(C.mean(dim=0), dt * np.arange(1, nspec + 1))
     peak = float(torch.max(spec.real).item())
 
     extra = {}
     if not args.no_check:
-        extra.update(accuracy_checks(cfg, prop, obs, total_steps, dt))
+        extra.update(accuracy_checks(cfg, prop, obs, total_steps, dt, R))
+    if not args.no_dense:
+        d = dense_operand_leg(cfg, prop.variant, 3)
+        extra["dense_operand"] = d
+        extra["dense_operand_rho_steps_per_s"] = d["value"]
     if not args.no_plain:
         extra.update(plain_fp64(cfg, B, dt, value))
 
+    traffic, traffic_note = profiled_traffic(args, prop.variant)
     result = {
         "metric": "RK4 rho-steps/s (batched FP32-MFMA Lindblad QRT propagation)",
         "value": value,
@@ -227,9 +263,9 @@ def main():
                    "flops_per_rho_step": fl},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA_TFLOPS,
-                     "traffic": profiled_traffic(args),
-                     "traffic_source": "profiles/lindblad_rk4_resident_lattice65_traffic.txt "
-                                       "(rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "traffic": traffic,
+                     "traffic_source": traffic_note + " (rocprofv3 FETCH_SIZE + WRITE_SIZE of "
+                                       "the same kernel source and variant, copy-calibrated)",
                      "kernel": f"lindblad_rk4_resident (bordered n=65, variant {prop.variant})",
                      "launch_ms_mean": float(np.mean(launch_ms)),
                      # FP32 MFMA work the kernel issues: the two complex products in the
@@ -251,30 +287,48 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def accuracy_checks(cfg, prop, obs, total_steps, dt):
-    """Max errors vs the FP64 oracle on a sample of the same workload."""
+def accuracy_checks(cfg, prop, obs, total_steps, dt, R):
+    """Max errors vs the FP64 oracle on the records of the timed run itself."""
     import torch
     from oracle import lindblad as ol
+    from oracle import spectra as osp
+    from lime_amd import fft as gfft
     from lime_amd.propagator import LindbladPropagator
     out = {}
-    nchk = min(256, total_steps)
-    # (1) C_a(t) of realisation 0 over the first nchk steps (from the timed run's records)
-    errs, scale = [], 0.0
-    for a in range(3):
-        _, ref, _ = ol.propagate(cfg["H"][0], cfg["x0"][0, a], cfg["c_ops"], dt, nchk, [cfg["mu"][0, a]])
-        got = obs[a, :nchk, 0].cpu().numpy()
-        errs.append(np.max(np.abs(got - ref[:, 0])))
-        scale = max(scale, np.max(np.abs(ref)))
+    nt = cfg["nt"]
+    nchk = min(nt, total_steps)
+    # (1) C_a(t) of the first and the last realisation, all three components, over the whole
+    #     nt-step trajectory of the workload, from the timed run's own records
+    errs, scale, spec_err, spec_scale = [], 0.0, 0.0, 0.0
+    t = dt * np.arange(1, nchk + 1)
+    for r in (0, R - 1):
+        iso_ref = np.zeros(nchk, dtype=np.complex128)
+        for a in range(3):
+            _, ref, _ = ol.propagate(cfg["H"][r], cfg["x0"][r, a], cfg["c_ops"], dt, nchk,
+                                     [cfg["mu"][r, a]])
+            got = obs[3 * r + a, :nchk, 0].cpu().numpy()
+            errs.append(np.max(np.abs(got - ref[:, 0])))
+            scale = max(scale, np.max(np.abs(ref)))
+            iso_ref += ref[:, 0] / 3.0
+        # (2) the realisation's spectrum: lime's transform (fft.py:9-56) of its isotropic
+        #     C(t), rocFFT on the GPU records against numpy FFT on the oracle's
+        iso_gpu = obs[3 * r:3 * r + 3, :nchk, 0].sum(dim=0) / 3.0
+        s_gpu, _ = gfft.fft(iso_gpu, t)
+        s_ref, _ = osp.fft(iso_ref, t)
+        spec_err = max(spec_err, float(np.max(np.abs(s_gpu.cpu().numpy() - s_ref))))
+        spec_scale = max(spec_scale, float(np.max(np.abs(s_ref))))
     out["qrt_max_abs_err"] = float(max(errs))
     out["qrt_max_rel_err"] = float(max(errs) / scale)
-    log(f"[check] C(t) max error {max(errs):.3e} (rel {max(errs) / scale:.3e}) over {nchk} steps")
-    # (2) density-matrix populations over the full trajectory for realisation 0:
+    out["qrt_checked"] = f"realisations 0 and {R - 1}, components x/y/z, records 1..{nchk}"
+    out["spectrum_max_rel_err"] = spec_err / spec_scale
+    log(f"[check] C(t) max error {max(errs):.3e} (rel {max(errs) / scale:.3e}) over {nchk} steps, "
+        f"realisations 0 and {R - 1}; spectrum rel error {spec_err / spec_scale:.3e}")
+    # (3) density-matrix populations over the full trajectory for realisation 0:
     #     rho0 = |mu_x><mu_x| / <mu_x|mu_x> in the site manifold (impulsive excitation)
     n = cfg["n"]
     v = cfg["mu"][0, 0][:, 0].copy()
     rho0 = np.outer(v, v.conj()) / np.vdot(v, v)
     P = [np.diag(np.eye(n)[i]).astype(np.complex128) for i in range(n)]
-    nt = cfg["nt"]
     p1 = LindbladPropagator(cfg["H"][0], cfg["c_ops"])
     rho, pobs, _ = p1.run(rho0, dt, nt, e_ops=P, nout=16)
     _, ref, _ = ol.propagate(cfg["H"][0], rho0, cfg["c_ops"], dt, nt, P, nout=16)
@@ -288,7 +342,56 @@ def accuracy_checks(cfg, prop, obs, total_steps, dt):
     return out
 
 
-def plain_fp64(cfg, B, dt, value, steps=6):
+def dense_operand_leg(cfg, variant, group, launches=2, chunk=256, seed=7):
+    """The headline kernel (same n, batch, launch shape and records) on dense data: GUE
+    H_eff sets of the lattice's spectral radius with on-site dephasing, and dense random
+    complex operators, so no operand is mostly zeros."""
+    import torch
+    from oracle import lindblad as ol
+    from lime_amd import configs
+    from lime_amd.propagator import LindbladPropagator
+    rng = np.random.default_rng(seed)
+    n = cfg["n"]
+    R = cfg["H"].shape[0]
+    B = group * R
+    radius = float(np.max(np.abs(np.linalg.eigvalsh(cfg["H"][0]))))
+    Hs = torch.as_tensor(configs.gue(rng, n, radius)).unsqueeze(0).repeat(R, 1, 1)
+    # per-realisation diagonal shift keeps the sets distinct without R eigendecompositions
+    Hs = Hs + torch.diag_embed(torch.as_tensor(rng.normal(0, 0.05, (R, n)).astype(np.complex128)))
+    prop = LindbladPropagator(Hs, cfg["c_ops"], group=group)
+    prop.variant = variant
+    dev = prop.dev
+    X0 = (rng.standard_normal((B, n, n)) + 1j * rng.standard_normal((B, n, n))) / n
+    X = torch.as_tensor(X0, device=dev).contiguous()
+    Ew = (rng.standard_normal((B, n, n)) + 1j * rng.standard_normal((B, n, n))) / n
+    E_t = torch.as_tensor(Ew, device=dev).unsqueeze(1).transpose(-1, -2).contiguous()
+    total = (launches + 1) * chunk
+    obs = torch.zeros((B, total, 1), dtype=torch.complex128, device=dev)
+    step = lambda i: prop.step(X, cfg["dt"], chunk, eops_t=E_t, eop_group=1, obs=obs, nout=1,
+                               step_offset=i * chunk, rec_capacity=total)
+    step(0)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
+    ev[0].record()
+    for i in range(launches):
+        step(1 + i)
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[-1]) / launches
+    value = B * chunk / (ms / 1e3)
+    tflops = flops_per_rho_step(n) * B * chunk / (ms / 1e3) / 1e12
+    # accuracy of one dense operator over the first launch
+    _, ref, _ = ol.propagate(Hs[0].numpy(), X0[0], cfg["c_ops"], cfg["dt"], chunk, [Ew[0]])
+    err = float(np.max(np.abs(obs[0, :chunk, 0].cpu().numpy() - ref[:, 0])) / np.max(np.abs(ref)))
+    log(f"[dense] dense operands: {value:.4g} rho-steps/s, {tflops:.2f} TFLOP/s "
+        f"({ms:.1f} ms per launch); C(t) rel err {err:.2e}")
+    return {"value": value, "unit": "rho-steps/s", "tflops": tflops,
+            "frac": tflops / PEAK_FP32_MFMA_TFLOPS, "launch_ms_mean": ms, "qrt_max_rel_err": err,
+            "data": f"{R} GUE H_eff sets (spectral radius {radius:.3f} + N(0, 0.05) site shifts), "
+                    f"on-site dephasing, {B} dense random complex operators and observables"}
+
+
+def plain_fp64(cfg, B, dt, value, steps=48):
     """The plain complex128 GPU propagator (batched zgemm + elementwise) on the same batch."""
     import torch
     from lime_amd.plain import PlainFP64Propagator
@@ -342,7 +445,8 @@ def _roofline(kernel, tflops, launch_ms, precision):
 
 
 def _dtype(precision):
-    return {"fp32": "c64-fp32mfma+f64", "3xbf16": "c64-3xbf16mfma+f64", "fp64": "c128-fp64mfma"}[precision]
+    return {"fp32": "c64-fp32mfma+f64", "3xbf16": "c64-3xbf16mfma+f64", "fp64": "c128-fp64mfma",
+            "fp32/fp64": "c64-fp32mfma+f64 forward, c128-fp64mfma adjoint"}[precision]
 
 
 def _cpu_sample(H, c_ops, X, dt, budget_s, label):
@@ -459,33 +563,48 @@ def bench_dense512(args):
 def bench_ragged(args):
     """2048 instances, n uniform in {16, 32, ..., 256}: GUE H (spectral radius 1), n on-site
     dephasing operators (gamma = 0.01) + 4 dense L_k (||L_k||_F = 0.1), rho0 = psi psi^dag,
-    dt = 0.02; all size classes replayed from one captured graph."""
+    dt = 0.02.  Default: every instance in ONE persistent launch (``RaggedLindblad``), the
+    launch captured in a hipGraph and replayed; ``--ragged-mode classes``: the size classes
+    on the per-class kernels captured into one graph (``RaggedClasses``)."""
     import torch
     from lime_amd import configs
-    from lime_amd.ragged import RaggedLindblad
+    from lime_amd.ragged import RaggedClasses, RaggedLindblad
     t0 = time.perf_counter()
     cfg = configs.ragged_gue(count=args.instances)
     log(f"[ragged] {args.instances} instances generated in {time.perf_counter() - t0:.1f} s")
-    rb = RaggedLindblad(cfg["instances"], precision=args.precision, tune=True, verbose=True)
+    persistent = args.ragged_mode == "persistent"
+    if persistent:
+        if args.precision != "fp32":
+            raise SystemExit("the persistent ragged kernel is built for FP32 MFMA (fp64 state); "
+                             "use --ragged-mode classes for other precisions")
+        rb = RaggedLindblad(cfg["instances"], tune=True, verbose=True)
+        shape = f"persistent launch, variant {rb.variant}"
+    else:
+        rb = RaggedClasses(cfg["instances"], precision=args.precision, tune=True, verbose=True)
+        shape = "hipGraph of per-class resident + tiled launches"
     chunk, dt = args.rk_per_step, cfg["dt"]
     rb.capture(dt, chunk)
     total_ms, per = _timed(lambda i: rb.graph.replay(), args.steps, args.warmup)
     count = len(cfg["instances"])
     value = count * chunk * args.steps / (total_ms / 1e3)
     tflops = rb.flops_per_step * chunk / (np.mean(per) / 1e3) / 1e12
-    log(f"[ragged] {value:.4g} rho-steps/s, {tflops:.1f} TFLOP/s canonical")
+    log(f"[ragged] {value:.4g} rho-steps/s, {tflops:.1f} TFLOP/s canonical ({shape})")
     sizes = np.asarray(cfg["sizes"])
+    config = {"workload": f"ragged_gue: {count} instances, n uniform in 16..256 step 16, dt=0.02, "
+                          f"one hipGraph replay = {chunk} RK4 steps of every instance",
+              "instances": count, "mean_n3": float(np.mean(sizes.astype(np.float64) ** 3)),
+              "rk_steps_per_replay": chunk, "seed": cfg["seed"],
+              "flops_per_batch_step": rb.flops_per_step, "mode": args.ragged_mode}
+    if persistent:
+        config["variant"] = rb.variant
+    else:
+        config["size_classes"] = len(rb.classes)
+        config["variants"] = {str(c["n"]): c["prop"].variant for c in rb.classes}
     out = _base(f"RK4 rho-steps/s (uneven batch n in 16..256, {args.precision})", value, args,
                 total_ms / args.steps, _dtype(args.precision),
-                "synthetic (seeded GUE instances, on-site dephasing + 4 dense L_k)",
-                {"workload": f"ragged_gue: {count} instances, n uniform in 16..256 step 16, dt=0.02, "
-                             f"one hipGraph replay = {chunk} RK4 steps of every instance",
-                 "instances": count, "size_classes": len(rb.classes),
-                 "mean_n3": float(np.mean(sizes.astype(np.float64) ** 3)),
-                 "rk_steps_per_replay": chunk, "seed": cfg["seed"],
-                 "flops_per_batch_step": rb.flops_per_step,
-                 "variants": {str(c["n"]): c["prop"].variant for c in rb.classes}},
-                _roofline("hipGraph of resident + tiled launches", tflops, float(np.mean(per)), args.precision))
+                "synthetic (seeded GUE instances, on-site dephasing + 4 dense L_k)", config,
+                _roofline("lindblad_ragged_persistent" if persistent else shape, tflops,
+                          float(np.mean(per)), args.precision))
     if not args.no_check:
         from oracle import lindblad as ol
         done = (args.warmup + args.steps) * chunk

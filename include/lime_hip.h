@@ -42,6 +42,10 @@ extern "C" {
                                         then complex128)                                  */
 #define LIME_F_FORCE_TILED 0x100   /* use the per-stage tiled kernels even when n fits the
                                       resident kernel (testing / tuning)                  */
+#define LIME_F_ONE_SIDED 0x200     /* drop the X H_eff^dag product: f(X) = -i H_eff X + D o X
+                                      + sum_k L_k X R_k^dag, i.e. dX/dt = G X with H_eff = i G
+                                      (Liouville-space generators acting on columns;
+                                      per-stage kernels)                                   */
 
 /*
  * A uniform batch of `batch` propagated operators, all of Hilbert dimension n.
@@ -108,6 +112,54 @@ typedef struct lime_step_params {
  */
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream);
 
+/*
+ * An uneven batch: `count` independent Lindblad systems of their own dimensions, packed.
+ * Each instance's operators are complex64 regions of `ops` (H_eff, optional Hadamard
+ * dissipator D, n_dense dense jump operators L_k, all n x n row-major), its state a
+ * complex128 region of `rho` (n x n, updated in place), and its per-stage operands a
+ * complex64 region of `work` of (3 + n_dense) n^2 elements.  Offsets count elements.
+ * The same right-hand side as lime_lindblad_batch with R_k = L_k.
+ */
+typedef struct lime_ragged_instance {
+  int32_t n;
+  int32_t n_dense;
+  int64_t heff;        /* offset of H_eff in ops                                        */
+  int64_t dmat;        /* offset of D in ops, or -1 (no diagonal jump operators)        */
+  int64_t lops;        /* offset of L_0 (L_k follow contiguously) in ops when n_dense > 0 */
+  int64_t rho;         /* offset of the state in rho                                    */
+  int64_t work;        /* offset of the instance's workspace in work                    */
+} lime_ragged_instance;
+
+typedef struct lime_ragged_batch {
+  int32_t count;                      /* instances                                       */
+  int32_t variant;                    /* launch shape, [0, lime_ragged_variants())       */
+  int32_t max_groups;                 /* cap on persistent workgroups (0 = fill the GPU) */
+  int32_t reserved;
+  const lime_ragged_instance* inst;   /* DEVICE array [count]                            */
+  const int32_t* order;               /* DEVICE array [count]: processing order, largest
+                                         n^3 (1 + n_dense) first                         */
+  const void* ops;                    /* complex64 operator pool                         */
+  void* rho;                          /* complex128 state pool                           */
+  void* work;                         /* complex64 workspace pool                        */
+  void* queue;                        /* DEVICE uint32 [2], zero before the first call;
+                                         the kernel leaves it zero when it ends          */
+} lime_ragged_batch;
+
+/*
+ * Advance every instance of an uneven batch by p->nsteps RK4 steps (p->dt) in ONE kernel
+ * launch: a persistent grid drains the instances in `order` through an atomic queue, each
+ * instance on one workgroup for all its steps.  Nothing is recorded (p->nout and
+ * rec_capacity are ignored); read the states from `rho`.  The caller guarantees that the
+ * offsets address its pools (the library does not read the device descriptor on the host).
+ * Capturable in a HIP graph (no host synchronisation, no allocation).
+ * Replaces lime's one-system-at-a-time loop: _lindblad (lime/oqs.py:1590-1688) called once
+ * per instance, each running rk4 (phys.py:636-649) over liouvillian (oqs.py:706-723).
+ */
+int lime_lindblad_rk4_ragged(const lime_ragged_batch* r, const lime_step_params* p, void* stream);
+
+/* Number of launch shapes of lime_lindblad_rk4_ragged (the autotuner times each one). */
+int lime_ragged_variants(void);
+
 /* Size of the static device scratch array of lime_lindblad_rk4_noalloc (per GPU). */
 #define LIME_NOALLOC_SCRATCH_BYTES (64ll << 20)
 
@@ -148,10 +200,27 @@ int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const voi
  *   out_b[j] += w (P_jj - (sum_m P_jm + sum_m P_mj)/2)  (Re(out_b) = d/d gamma_j)
  *   out_u[j] += w (coupling contraction of H_j,j+1 = H_j+1,j) (Im(out_u))
  * out_a, out_b: complex128 [batch][n]; out_u: complex128 [batch][n-1].
+ * Deterministic (fixed summation order, no atomics).
  * No counterpart in lime (it has no gradients).
  */
 int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const void* s, double w,
                           void* out_a, void* out_b, void* out_u, void* stream);
+
+/* The same with P = sum_p conj(kappa[p]) o s[p] over `pairs` (1..4) operator pairs in one
+ * launch (`kappa`, `s`: HOST arrays of device pointers): the four stage contractions of an
+ * RK4 step.  No counterpart in lime. */
+int lime_adjoint_contract_pairs(int32_t batch, int32_t n, int32_t pairs, const void* const* kappa,
+                                const void* const* s, double w, void* out_a, void* out_b, void* out_u,
+                                void* stream);
+
+/*
+ * Start of one backward RK4 step of the adjoint, elementwise over complex128 [batch][n][n]:
+ *   lam += g[b] m,  k4 = c lam,  k4_op = k4 in the operand precision of `flags`
+ *   (complex64, or complex128 with LIME_F_PREC_FP64),  acc = lam.
+ * g: complex128 [batch]; m: complex128 [n][n] shared by the batch.  No counterpart in lime.
+ */
+int lime_adjoint_seed(int32_t batch, int32_t n, int32_t flags, void* lam, const void* g, const void* m,
+                      double c, void* k4, void* k4_op, void* acc, void* stream);
 
 /* Largest Hilbert dimension the resident (workspace-free) kernel accepts. */
 int lime_lindblad_max_n(int32_t n_dense, int32_t flags);

@@ -20,17 +20,22 @@ LIME_F_PREC_FP32 = 0x00
 LIME_F_PREC_3XBF16 = 0x10
 LIME_F_PREC_FP64 = 0x20
 LIME_F_FORCE_TILED = 0x100
+LIME_F_ONE_SIDED = 0x200
 LIME_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 #: every symbol include/lime_hip.h declares (checked by tests/test_abi.py)
 EXPORTS = (
     "lime_lindblad_rk4",
     "lime_lindblad_rk4_noalloc",
+    "lime_lindblad_rk4_ragged",
+    "lime_ragged_variants",
     "lime_lindblad_max_n",
     "lime_lindblad_workspace_bytes",
     "lime_lindblad_variants",
     "lime_lindblad_apply",
     "lime_adjoint_contract",
+    "lime_adjoint_contract_pairs",
+    "lime_adjoint_seed",
     "lime_last_error",
     "lime_version",
 )
@@ -72,6 +77,33 @@ class LindbladBatch(ctypes.Structure):
     ]
 
 
+class RaggedInstance(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("n_dense", ctypes.c_int32),
+        ("heff", ctypes.c_int64),
+        ("dmat", ctypes.c_int64),
+        ("lops", ctypes.c_int64),
+        ("rho", ctypes.c_int64),
+        ("work", ctypes.c_int64),
+    ]
+
+
+class RaggedBatch(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
+        ("max_groups", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("inst", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("ops", ctypes.c_void_p),
+        ("rho", ctypes.c_void_p),
+        ("work", ctypes.c_void_p),
+        ("queue", ctypes.c_void_p),
+    ]
+
+
 class StepParams(ctypes.Structure):
     _fields_ = [
         ("dt", ctypes.c_double),
@@ -103,6 +135,11 @@ def load(path=LIB_PATH):
         lib.lime_lindblad_rk4_noalloc.argtypes = [ctypes.POINTER(LindbladBatch),
                                                   ctypes.POINTER(StepParams), ctypes.c_void_p]
         lib.lime_lindblad_rk4_noalloc.restype = ctypes.c_int
+        lib.lime_lindblad_rk4_ragged.argtypes = [ctypes.POINTER(RaggedBatch),
+                                                 ctypes.POINTER(StepParams), ctypes.c_void_p]
+        lib.lime_lindblad_rk4_ragged.restype = ctypes.c_int
+        lib.lime_ragged_variants.argtypes = []
+        lib.lime_ragged_variants.restype = ctypes.c_int
         lib.lime_lindblad_max_n.argtypes = [ctypes.c_int32, ctypes.c_int32]
         lib.lime_lindblad_max_n.restype = ctypes.c_int
         lib.lime_lindblad_workspace_bytes.argtypes = [ctypes.POINTER(LindbladBatch)]
@@ -116,6 +153,14 @@ def load(path=LIB_PATH):
         lib.lime_adjoint_contract.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp,
                                               ctypes.c_double, vp, vp, vp, vp]
         lib.lime_adjoint_contract.restype = ctypes.c_int
+        lib.lime_adjoint_contract_pairs.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                    ctypes.POINTER(ctypes.c_void_p),
+                                                    ctypes.POINTER(ctypes.c_void_p), ctypes.c_double,
+                                                    vp, vp, vp, vp]
+        lib.lime_adjoint_contract_pairs.restype = ctypes.c_int
+        lib.lime_adjoint_seed.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp,
+                                          ctypes.c_double, vp, vp, vp, vp]
+        lib.lime_adjoint_seed.restype = ctypes.c_int
         lib.lime_last_error.argtypes = []
         lib.lime_last_error.restype = ctypes.c_char_p
         lib.lime_version.argtypes = []

@@ -26,6 +26,8 @@ s_2, s_3, s_4 in FP64 as the step computes them) and lambda' = lambda_{k+1}:
 f^dag is the generator with H_eff -> -H_eff^dag, D -> conj(D): the SAME kernels
 (``lime_lindblad_apply``); the contractions run in ``lime_adjoint_contract``.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -63,7 +65,10 @@ class ChainQRT:
     """Forward propagation with checkpoints and the adjoint backward pass."""
 
     def __init__(self, eps, J, gamma, mu_sites, dt, nt, checkpoint=64, force_tiled=False,
-                 precision="fp32", tune=True):
+                 precision="fp32", tune=True, backward_precision=None):
+        """``precision``: arithmetic of the forward pass and of the recomputation of the
+        stage operands; ``backward_precision`` (default: the same): arithmetic of the
+        adjoint generator applications f^dag ('fp32' or 'fp64')."""
         self.dev = device()
         self.eps = eps.to(self.dev, torch.float64)
         self.J = J.to(self.dev, torch.float64)
@@ -78,7 +83,9 @@ class ChainQRT:
         if tune:
             from . import autotune
             autotune.tune(self.prop, self.x0, self.dt)
-        self.adj = self.prop.adjoint()
+        bp = backward_precision or precision
+        self.adj = (self.prop if bp == precision else
+                    LindbladPropagator.from_folded(heff, dmat, force_tiled=True, precision=bp)).adjoint()
         # the backward recompute records the stage operands on the per-stage kernels
         self.rec_prop = self.prop.tiled()
         self.mu = dipole_operator(torch.as_tensor(mu_sites, device=self.dev), self.n)
@@ -135,35 +142,43 @@ class ChainQRT:
         outb = torch.zeros((B, n), dtype=torch.complex128, device=dev)
         outu = torch.zeros((B, max(n - 1, 1)), dtype=torch.complex128, device=dev)
         c128 = lambda: torch.empty((B, n, n), dtype=torch.complex128, device=dev)
-        opd = self.prop.operand_dtype
+        opd = self.adj.operand_dtype
         c64 = lambda: torch.empty((B, n, n), dtype=opd, device=dev)
         k3, k2, k1 = c128(), c128(), c128()
         k3l, k2l, k1l = c64(), c64(), c64()
-        lib = self.prop._lib
+        mu_h = mu_h.contiguous()
+        lib = self.adj._lib
         from . import _lib as L
         stream = torch.cuda.current_stream(dev)
+        sp = stream.cuda_stream
+        acc = c128()
+        k4 = c128()
+        k4l = c64()
+        flags = L.LIME_F_PREC_FP64 if opd == torch.complex128 else L.LIME_F_PREC_FP32
+        pairs = ctypes.c_void_p * 4
         nseg = (self.nt + self.K - 1) // self.K
         for seg in range(nseg - 1, -1, -1):
             c0 = seg * self.K
             length = min(self.K, self.nt - c0)
             Xs, stages = self._segment_states(c0, length)
             for k in range(c0 + length - 1, c0 - 1, -1):
-                # lambda' = lambda_{k+1}: add the source of record k+1 (C_{k+1} = Tr[mu X_{k+1}])
-                lam.addcmul_(Gt[k].view(B, 1, 1), mu_h)
+                # lambda' = lambda_{k+1} + source of record k+1 (C_{k+1} = Tr[mu X_{k+1}]);
+                # kappa_4 = h/6 lambda' (FP64 and operand precision); acc = lambda'
+                L.check(lib.lime_adjoint_seed(B, n, flags, lam.data_ptr(), Gt[k].data_ptr(),
+                                              mu_h.data_ptr(), h / 6.0, k4.data_ptr(), k4l.data_ptr(),
+                                              acc.data_ptr(), sp))
                 X = Xs[k - c0]
                 s2, s3, s4 = stages[k - c0]                      # contiguous views
-                k4 = (h / 6.0) * lam
-                k4l = k4.to(opd).contiguous()
-                acc = lam.clone()
                 self.adj.apply(k4l, z=lam, y=k3, y64=k3l, acc=acc, c_lx=h, c_z=h / 3.0, c_acc=1.0)
                 self.adj.apply(k3l, z=lam, y=k2, y64=k2l, acc=acc, c_lx=h / 2, c_z=h / 3.0, c_acc=1.0)
                 self.adj.apply(k2l, z=lam, y=k1, y64=k1l, acc=acc, c_lx=h / 2, c_z=h / 6.0, c_acc=1.0)
                 self.adj.apply(k1l, acc=acc, c_acc=1.0)
-                for kap, s in ((k1, X), (k2, s2), (k3, s3), (k4, s4)):
-                    L.check(lib.lime_adjoint_contract(B, n, kap.data_ptr(), s.data_ptr(), 1.0,
-                                                      outa.data_ptr(), outb.data_ptr(),
-                                                      outu.data_ptr(), stream.cuda_stream))
-                lam = acc
+                # the four stage contractions in one launch
+                kap = pairs(k1.data_ptr(), k2.data_ptr(), k3.data_ptr(), k4.data_ptr())
+                sts = pairs(X.data_ptr(), s2.data_ptr(), s3.data_ptr(), s4.data_ptr())
+                L.check(lib.lime_adjoint_contract_pairs(B, n, 4, kap, sts, 1.0, outa.data_ptr(),
+                                                        outb.data_ptr(), outu.data_ptr(), sp))
+                lam, acc = acc, lam                      # lambda_k = acc; the old buffer is reused
         g_eps = outa[:, 1:].imag.clone()
         g_J = outu[:, 1:].imag.clone()
         g_gamma = outb[:, 1:].real.sum(dim=1)
@@ -176,13 +191,15 @@ class ChainAbsorption(torch.autograd.Function):
     forward(eps (B, ns), J (B, ns-1), gamma (scalar or (B,)), mu_sites (ns,), dt, nt, K,
             precision) -> C (B, nt) complex128.  gamma's gradient is summed over the batch
     when gamma is a scalar (a shared rate).  precision 'fp32' runs the FP32-MFMA kernels,
-    'fp64' the FP64-MFMA tiled kernels for both passes.
+    'fp64' the FP64-MFMA tiled kernels for both passes, 'fp32/fp64' the FP32-MFMA forward
+    and recomputation with FP64-MFMA adjoint applications.
     """
 
     @staticmethod
     def forward(ctx, eps, J, gamma, mu_sites, dt, nt, checkpoint=64, precision="fp32"):
+        fwd, _, bwd = precision.partition("/")
         q = ChainQRT(eps.detach(), J.detach(), gamma.detach(), mu_sites, dt, nt, checkpoint,
-                     precision=precision)
+                     precision=fwd, backward_precision=bwd or fwd)
         C = q.forward()
         ctx.q = q
         ctx.gamma_shape = gamma.shape

@@ -6,7 +6,8 @@ each padded size, for the tiled kernels the 32x32 or 64x64 output tile.  ``tune`
 every variant on the caller's own batch for a few RK4 steps (after one warm-up launch,
 HIP events on the launch stream), keeps the fastest, and caches the choice per
 (n, dense operators, precision, batch bucket, observables) in a JSON file
-(``LIME_TUNE_CACHE`` or ``lime_amd/_tune_cache.json``).
+(``LIME_TUNE_CACHE``, else ``$XDG_CACHE_HOME/lime_amd/tune.json`` or
+``~/.cache/lime_amd/tune.json``; never inside the package).
 """
 import json
 import os
@@ -16,8 +17,9 @@ import threading
 import torch
 
 _lock = threading.Lock()
-_CACHE_PATH = os.environ.get("LIME_TUNE_CACHE",
-                             os.path.join(os.path.dirname(os.path.abspath(__file__)), "_tune_cache.json"))
+_CACHE_PATH = os.environ.get("LIME_TUNE_CACHE") or os.path.join(
+    os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache"),
+    "lime_amd", "tune.json")
 _cache = None
 
 
@@ -34,6 +36,7 @@ def _load():
 
 def _save():
     try:
+        os.makedirs(os.path.dirname(_CACHE_PATH), exist_ok=True)
         tmp = _CACHE_PATH + ".tmp"
         with open(tmp, "w") as f:
             json.dump(_cache, f, indent=1, sort_keys=True)
@@ -101,5 +104,38 @@ def tune(prop, rho, dt, steps=4, n_eops=0, force=False, verbose=False, eops_t=No
               + f" -> v{best}", file=sys.stderr)
     with _lock:
         _load()[key_of(prop, n_eops)] = {"variant": best, "ms": {str(v): t for v, t in times.items()}}
+        _save()
+    return best, times
+
+
+def tune_ragged(rb, dt, steps=2, force=False, verbose=False):
+    """Pick the fastest persistent launch shape of a ``RaggedLindblad`` batch (states are
+    restored afterwards); cached per (instance count, sum of n^3 (1 + K_d))."""
+    key = f"ragged:c{rb.count}:w{int(rb.work_units)}"
+    with _lock:
+        cache = _load()
+        if not force and key in cache:
+            rb.variant = int(cache[key]["variant"])
+            return rb.variant, cache[key].get("ms", {})
+    saved = rb.rho.clone()
+    times = {}
+    stream = torch.cuda.current_stream(rb.dev)
+    for v in range(rb.variants()):
+        rb.variant = v
+        rb.step(dt, 1)                                   # warm-up
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rb.step(dt, steps)
+        e1.record(stream)
+        torch.cuda.synchronize(rb.dev)
+        times[v] = e0.elapsed_time(e1) / steps
+    rb.rho.copy_(saved)
+    best = min(times, key=times.get)
+    rb.variant = best
+    if verbose:
+        print(f"[tune] {key}: " + ", ".join(f"v{v}={t:.3f}ms" for v, t in sorted(times.items()))
+              + f" -> v{best}", file=sys.stderr)
+    with _lock:
+        _load()[key] = {"variant": best, "ms": {str(v): t for v, t in times.items()}}
         _save()
     return best, times

@@ -1,4 +1,5 @@
 // extern "C" entry points of liblime_hip.so (declared in include/lime_hip.h).
+#include <cstddef>
 #include <cstdio>
 #include <mutex>
 #include <string>
@@ -7,6 +8,11 @@
 #include "lime_hip.h"
 #include "lindblad_resident_decl.hpp"
 #include "lindblad_tiled_decl.hpp"
+
+static_assert(LIME_F_ONE_SIDED == lime::kFlagOneSided, "one flag value");
+static_assert(sizeof(lime_ragged_instance) == sizeof(lime::RaggedInstance) &&
+                  offsetof(lime_ragged_instance, work) == offsetof(lime::RaggedInstance, work),
+              "the ABI instance record is the kernel's");
 
 namespace {
 
@@ -36,7 +42,7 @@ bool is_split_bf16(const lime_lindblad_batch* d) {
 }
 
 bool use_resident(const lime_lindblad_batch* d) {
-  if (d->flags & LIME_F_FORCE_TILED) return false;
+  if (d->flags & (LIME_F_FORCE_TILED | LIME_F_ONE_SIDED)) return false;
   // the FP64 and 3xBF16 variants are built for the tiled kernels
   if ((d->flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32) return false;
   const int nb = lime::cdiv(d->n, 16);
@@ -134,8 +140,8 @@ int rk4_impl(const lime_lindblad_batch* d, const lime_step_params* p, void* stre
   }
 
   // ---- tiled path: one launch per RK4 stage on the workspace
-  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
-                                                 (d->flags & LIME_F_RENORM_TRACE) != 0, d->obs ? d->n_eops : 0);
+  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, d->flags, is_fp64(d),
+                                                 d->obs ? d->n_eops : 0);
   if (noalloc && need > kScratchBytes)
     return fail(LIME_ERR_ARG, std::string(name) + ": this batch needs " + std::to_string(need) +
                                   " bytes of scratch, above the static " + std::to_string(kScratchBytes) +
@@ -199,8 +205,8 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
   if (use_resident(d)) return 0;
-  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, is_fp64(d),
-                                             (d->flags & LIME_F_RENORM_TRACE) != 0, d->obs ? d->n_eops : 0);
+  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, d->flags, is_fp64(d),
+                                             d->obs ? d->n_eops : 0);
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {
@@ -232,6 +238,7 @@ int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const voi
   a.n = d->n;
   a.group = d->group;
   a.n_dense = d->n_dense;
+  a.flags = d->flags;
   a.heff = d->heff;
   a.dmat = d->dmat;
   a.lops = d->lops;
@@ -250,25 +257,84 @@ int lime_lindblad_apply(const lime_lindblad_batch* d, const void* x64, const voi
   return LIME_OK;
 }
 
-int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const void* s, double w,
-                          void* out_a, void* out_b, void* out_u, void* stream) {
+int lime_ragged_variants(void) { return lime::ragged_variants(); }
+
+int lime_lindblad_rk4_ragged(const lime_ragged_batch* r, const lime_step_params* p, void* stream) {
+  if (r == nullptr || p == nullptr) return fail(LIME_ERR_ARG, "lime_lindblad_rk4_ragged: null descriptor");
+  if (r->count < 0 || p->nsteps < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4_ragged: count and nsteps must be >= 0");
+  if (r->count == 0 || p->nsteps == 0) return LIME_OK;
+  if (!r->inst || !r->order || !r->ops || !r->rho || !r->work || !r->queue)
+    return fail(LIME_ERR_ARG, "lime_lindblad_rk4_ragged: inst, order, ops, rho, work and queue are required");
+  if (r->variant < 0 || r->variant >= lime::ragged_variants())
+    return fail(LIME_ERR_ARG, "lime_lindblad_rk4_ragged: variant out of range");
+  if (r->max_groups < 0) return fail(LIME_ERR_ARG, "lime_lindblad_rk4_ragged: max_groups < 0");
+  lime::RaggedArgs a;
+  a.inst = reinterpret_cast<const lime::RaggedInstance*>(r->inst);
+  a.order = r->order;
+  a.count = r->count;
+  a.nsteps = p->nsteps;
+  a.dt = p->dt;
+  a.ops = static_cast<const float2*>(r->ops);
+  a.rho = static_cast<double2*>(r->rho);
+  a.work = static_cast<float2*>(r->work);
+  a.queue = static_cast<unsigned*>(r->queue);
+  hipError_t e = lime::ragged_run(a, r->variant, r->max_groups, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "lime_lindblad_rk4_ragged launch");
+  return LIME_OK;
+}
+
+int lime_adjoint_contract_pairs(int32_t batch, int32_t n, int32_t pairs, const void* const* kappa,
+                                const void* const* s, double w, void* out_a, void* out_b, void* out_u,
+                                void* stream) {
   if (batch < 0 || n < 1) return fail(LIME_ERR_ARG, "lime_adjoint_contract: bad batch/n");
+  if (pairs < 1 || pairs > lime::kMaxContractPairs)
+    return fail(LIME_ERR_ARG, "lime_adjoint_contract_pairs: pairs must be in 1..4");
   if (lime::contract_lds_bytes(n) > 160 * 1024)
     return fail(LIME_ERR_UNSUPPORTED, "lime_adjoint_contract: n > 930 is not built");
   if (batch == 0) return LIME_OK;
   if (!kappa || !s || !out_a || !out_b || (n > 1 && !out_u))
     return fail(LIME_ERR_ARG, "lime_adjoint_contract: null pointer");
-  lime::ContractArgs c;
+  lime::ContractArgs c = {};
   c.batch = batch;
   c.n = n;
-  c.kappa = static_cast<const double2*>(kappa);
-  c.s = static_cast<const double2*>(s);
+  c.pairs = pairs;
+  for (int p = 0; p < pairs; ++p) {
+    if (!kappa[p] || !s[p]) return fail(LIME_ERR_ARG, "lime_adjoint_contract: null pointer");
+    c.kappa[p] = static_cast<const double2*>(kappa[p]);
+    c.s[p] = static_cast<const double2*>(s[p]);
+  }
   c.weight = w;
   c.out_a = static_cast<double2*>(out_a);
   c.out_b = static_cast<double2*>(out_b);
   c.out_u = static_cast<double2*>(out_u);
   hipError_t e = lime::contract_launch(c, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "lime_adjoint_contract launch");
+  return LIME_OK;
+}
+
+int lime_adjoint_contract(int32_t batch, int32_t n, const void* kappa, const void* s, double w,
+                          void* out_a, void* out_b, void* out_u, void* stream) {
+  return lime_adjoint_contract_pairs(batch, n, 1, &kappa, &s, w, out_a, out_b, out_u, stream);
+}
+
+int lime_adjoint_seed(int32_t batch, int32_t n, int32_t flags, void* lam, const void* g, const void* m,
+                      double c, void* k4, void* k4_op, void* acc, void* stream) {
+  if (batch < 0 || n < 1) return fail(LIME_ERR_ARG, "lime_adjoint_seed: bad batch/n");
+  if (batch == 0) return LIME_OK;
+  if (!lam || !g || !m || !k4 || !k4_op || !acc) return fail(LIME_ERR_ARG, "lime_adjoint_seed: null pointer");
+  lime::SeedArgs a;
+  a.batch = batch;
+  a.n = n;
+  a.op_fp64 = (flags & LIME_F_PREC_MASK) == LIME_F_PREC_FP64;
+  a.lam = static_cast<double2*>(lam);
+  a.g = static_cast<const double2*>(g);
+  a.m = static_cast<const double2*>(m);
+  a.c = c;
+  a.k4 = static_cast<double2*>(k4);
+  a.k4_op = k4_op;
+  a.acc = static_cast<double2*>(acc);
+  hipError_t e = lime::seed_launch(a, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "lime_adjoint_seed launch");
   return LIME_OK;
 }
 

@@ -138,6 +138,9 @@ lindblad_rk4_resident(ResidentArgs a) {
   // held as cr - di (Hx Sx in cr, Sx Hx' in di: no negated operand in the MFMA stream), R2
   // in dr, M1 in ci.  A value (cr0, ci0) of C enters them as cr = cr0, ci = ci0 + cr0.
   constexpr bool G3 = SPLIT && !DENSE;
+  // PF: the H_eff / S fragments of k-step k0 + 4 are read while step k0's MFMAs issue (one
+  // tile per wave; with more, the tiles' MFMAs interleave and the prefetch registers spill)
+  constexpr bool PF = BPW == 1;
   // DB: double-buffered stage operand (when it fits the LDS; never with dense operators):
   // the epilogue writes the next S into the other buffer, so a stage needs one barrier
   static_assert(!(DB && DENSE), "the double buffer is built without dense operators");
@@ -380,9 +383,20 @@ lindblad_rk4_resident(ResidentArgs a) {
           // bordered: MFMA steps k0 < NC only; the step k0 = NC carries the border terms
           const bool mstep = !BRD || k0 < NC;
           float2 xn[BPW], yn[BPW], zn[BPW], hn[BPW];
+          if constexpr (!PF) {
+            // several tiles per wave interleave their MFMAs: the fragments of this step are
+            // read here instead of one step ahead (their registers would spill)
+#pragma unroll
+            for (int u = 0; u < BPW; ++u) {
+              x[u] = sH[oR[u] + k0];
+              y[u] = sS[oC[u] + k0 * P];
+              z[u] = sS[oR[u] + k0];
+              h[u] = sH[oHc[u] + k0];
+            }
+          }
           // prefetch step k0+4; at k0+4 == ke this reads padding or spare rows, discarded
 #pragma unroll
-          for (int u = 0; u < BPW; ++u) {
+          for (int u = 0; u < BPW && PF; ++u) {
             if (!BRD || k0 + 4 < NC) {
               xn[u] = sH[oR[u] + k0 + 4];
               yn[u] = sS[oC[u] + (k0 + 4) * P];
@@ -443,7 +457,7 @@ lindblad_rk4_resident(ResidentArgs a) {
             }
           }
 #pragma unroll
-          for (int u = 0; u < BPW; ++u) {
+          for (int u = 0; u < BPW && PF; ++u) {
             x[u] = xn[u];
             y[u] = yn[u];
             z[u] = zn[u];
@@ -779,20 +793,20 @@ constexpr int kMaxVariants = 4;
 constexpr Shape kShapes[7][kMaxVariants] = {
     {},
     {{1, 1}, {0, 0}, {0, 0}, {0, 0}},
-    {{2, 2}, {4, 1}, {1, 4}, {0, 0}},
-    {{3, 3}, {9, 1}, {5, 2}, {0, 0}},
-    {{8, 2}, {4, 4}, {16, 1}, {6, 3}},
-    {{13, 2}, {9, 3}, {7, 4}, {0, 0}},
+    {{4, 1}, {2, 2}, {0, 0}, {0, 0}},
+    {{9, 1}, {5, 2}, {0, 0}, {0, 0}},
+    {{16, 1}, {8, 2}, {0, 0}, {0, 0}},
+    {{13, 2}, {0, 0}, {0, 0}, {0, 0}},
     {{12, 3}, {0, 0}, {0, 0}, {0, 0}},
 };
 constexpr Shape kBorderShapes[7][kMaxVariants] = {
     {},
     {},
     {{4, 1}, {2, 1}, {0, 0}, {0, 0}},
-    {{4, 1}, {8, 1}, {2, 2}, {0, 0}},
-    {{9, 1}, {5, 2}, {3, 3}, {0, 0}},
-    {{16, 1}, {8, 2}, {4, 4}, {0, 0}},
-    {{13, 2}, {9, 3}, {7, 4}, {0, 0}},
+    {{4, 1}, {8, 1}, {0, 0}, {0, 0}},
+    {{9, 1}, {0, 0}, {0, 0}, {0, 0}},
+    {{16, 1}, {0, 0}, {0, 0}, {0, 0}},
+    {{13, 2}, {0, 0}, {0, 0}, {0, 0}},
 };
 
 template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD, bool DB>
@@ -824,8 +838,11 @@ template <int NB, int W, int BPW>
 hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream, bool db) {
   static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
   constexpr int MINW = BPW == 1 ? 6 : BPW == 2 ? 4 : BPW == 3 ? 3 : 2;
+  // the dense products stage T = L S and their own accumulators: one wave per SIMD fewer
+  constexpr int MINW_D = (BPW == 1 ? MINW - 2 : MINW - 1) > (W + 3) / 4 ? (BPW == 1 ? MINW - 2 : MINW - 1)
+                                                                         : (W + 3) / 4;
   if constexpr (NB <= 4) {
-    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW, true, false>(a, lds, stream, false);
+    if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW_D, true, false>(a, lds, stream, false);
   } else {
     if (a.n_dense > 0) return hipErrorInvalidValue;
   }
@@ -897,34 +914,21 @@ hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t strea
       case 21: return launch_border<2, 2, 1>(a, lds, stream, db);
       case 30: return launch_border<3, 4, 1>(a, lds, stream, db);
       case 31: return launch_border<3, 8, 1>(a, lds, stream, db);
-      case 32: return launch_border<3, 2, 2>(a, lds, stream, db);
       case 40: return launch_border<4, 9, 1>(a, lds, stream, db);
-      case 41: return launch_border<4, 5, 2>(a, lds, stream, db);
-      case 42: return launch_border<4, 3, 3>(a, lds, stream, db);
       case 50: return launch_border<5, 16, 1>(a, lds, stream, db);
-      case 51: return launch_border<5, 8, 2>(a, lds, stream, db);
-      case 52: return launch_border<5, 4, 4>(a, lds, stream, db);
       case 60: return launch_border<6, 13, 2>(a, lds, stream, db);
-      case 61: return launch_border<6, 9, 3>(a, lds, stream, db);
-      case 62: return launch_border<6, 7, 4>(a, lds, stream, db);
       default: return hipErrorInvalidValue;
     }
   }
   switch (nb * 10 + variant) {
     case 10: return launch_shape<1, 1, 1>(a, lds, stream, db);
-    case 20: return launch_shape<2, 2, 2>(a, lds, stream, db);
-    case 21: return launch_shape<2, 4, 1>(a, lds, stream, db);
-    case 22: return launch_shape<2, 1, 4>(a, lds, stream, db);
-    case 30: return launch_shape<3, 3, 3>(a, lds, stream, db);
-    case 31: return launch_shape<3, 9, 1>(a, lds, stream, db);
-    case 32: return launch_shape<3, 5, 2>(a, lds, stream, db);
-    case 40: return launch_shape<4, 8, 2>(a, lds, stream, db);
-    case 41: return launch_shape<4, 4, 4>(a, lds, stream, db);
-    case 42: return launch_shape<4, 16, 1>(a, lds, stream, db);
-    case 43: return launch_shape<4, 6, 3>(a, lds, stream, db);
+    case 20: return launch_shape<2, 4, 1>(a, lds, stream, db);
+    case 21: return launch_shape<2, 2, 2>(a, lds, stream, db);
+    case 30: return launch_shape<3, 9, 1>(a, lds, stream, db);
+    case 31: return launch_shape<3, 5, 2>(a, lds, stream, db);
+    case 40: return launch_shape<4, 16, 1>(a, lds, stream, db);
+    case 41: return launch_shape<4, 8, 2>(a, lds, stream, db);
     case 50: return launch_shape<5, 13, 2>(a, lds, stream, db);
-    case 51: return launch_shape<5, 9, 3>(a, lds, stream, db);
-    case 52: return launch_shape<5, 7, 4>(a, lds, stream, db);
     case 60: return launch_shape<6, 12, 3>(a, lds, stream, db);
     default: return hipErrorInvalidValue;
   }

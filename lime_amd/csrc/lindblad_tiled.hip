@@ -354,13 +354,15 @@ struct Geo {
   int row0, col0, tid, lane, w, wr, wc, lr, lg;
 };
 
+// tid: the thread's index within its group of 256 threads (the workgroup, or one tile
+// group of the persistent ragged kernel).
 template <int TILE>
-__device__ __forceinline__ Geo geometry(int n, int tile_index) {
+__device__ __forceinline__ Geo geometry(int n, int tile_index, int tid) {
   Geo g;
   const int tpd = tiles_per_dim(n, TILE);
   g.row0 = (tile_index / tpd) * TILE;
   g.col0 = (tile_index % tpd) * TILE;
-  g.tid = threadIdx.x;
+  g.tid = tid;
   g.lane = g.tid & 63;
   g.w = __builtin_amdgcn_readfirstlane(g.tid >> 6);
   g.wr = g.w >> 1;
@@ -387,7 +389,9 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
 #pragma unroll
     for (int q = 0; q < TILE / 32; ++q) dr[p][q] = V4{0, 0, 0, 0};
   term<TILE, PLAIN, R, P>(H, S, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
-  term<TILE, NEG_AH, R, P>(S, H, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+  // one-sided generators (dX/dt = G X, H_eff = i G) have no X H_eff^dag product
+  if (!(a.flags & kFlagOneSided))
+    term<TILE, NEG_AH, R, P>(S, H, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
   for (int k = 0; k < a.n_dense; ++k) {
     const C* T = static_cast<const C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
     const C* Rk = static_cast<const C*>(a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + k) * nn;
@@ -398,17 +402,11 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
 
 }  // namespace
 
-// T[b][k] = L[set][k] @ S[b]  for every dense operator k.
+// One TILE x TILE tile of T[b][k] = L[set][k] @ S[b] (set = b / group).
 template <int TILE, typename R, int P>
-__global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
+__device__ __forceinline__ void left_tile(const TiledArgs& a, int b, int k, const Geo& g, unsigned char* smem) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
-  __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
-  const int per = tiles_per_dim(a.n, TILE) * tiles_per_dim(a.n, TILE);
-  const int blk = xcd_block();
-  const int bk = blk / per;                   // = b * n_dense + k
-  const Geo g = geometry<TILE>(a.n, blk % per);
-  const int b = bk / a.n_dense, k = bk % a.n_dense;
   const int set = b / a.group;
   const size_t nn = (size_t)a.n * a.n;
   constexpr int BW = TILE / 32;
@@ -432,6 +430,17 @@ __global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
         const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
         if (i < a.n && j < a.n) T[(size_t)i * a.n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
       }
+}
+
+// T[b][k] = L[set][k] @ S[b]  for every dense operator k.
+template <int TILE, typename R, int P>
+__global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
+  __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
+  const int per = tiles_per_dim(a.n, TILE) * tiles_per_dim(a.n, TILE);
+  const int blk = xcd_block();
+  const int bk = blk / per;                   // = b * n_dense + k
+  const Geo g = geometry<TILE>(a.n, blk % per, threadIdx.x);
+  left_tile<TILE, R, P>(a, bk / a.n_dense, bk % a.n_dense, g, smem);
 }
 
 // ---- per-element epilogues shared by the MFMA tiles and the border workgroups
@@ -623,7 +632,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
     return;
   }
 
-  const Geo g = geometry<TILE>(nc, idx);
+  const Geo g = geometry<TILE>(nc, idx, threadIdx.x);
   constexpr int BW = TILE / 32;
   V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
@@ -694,7 +703,7 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
     return;
   }
 
-  const Geo g = geometry<TILE>(nc, idx);
+  const Geo g = geometry<TILE>(nc, idx, threadIdx.x);
   constexpr int BW = TILE / 32;
   V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
@@ -713,6 +722,121 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
         if (i >= nc || j >= nc) continue;
         apply_elem<R>(a, b, set, (size_t)i * n + j, cr[p][q][v], ci[p][q][v]);
       }
+}
+
+// ---- uneven batches in one persistent launch
+// A fixed grid of workgroups drains a queue of instances ordered by cost (largest n^3 (1 +
+// K_d) first): a workgroup takes the next instance with one atomic and advances it through
+// every RK4 step of the call before taking another, so the largest systems start first and
+// the small ones fill the tail.  Each workgroup holds SG groups of 256 threads; every group
+// runs the TILE x TILE tile machinery of the per-stage kernels (K-panels staged through its
+// own LDS area) on tiles t, t + SG, ... of the instance, and a workgroup barrier separates
+// the dense products T_k = L_k S of a stage, the stage's tiles, and the next stage.  An
+// instance never leaves its workgroup, so every operand it writes is read back by the same
+// compute unit: no synchronisation between workgroups and no cross-XCD coherence traffic.
+// The last workgroup to finish resets the queue, so the next launch (or graph replay)
+// starts from an empty queue with no host work.
+__device__ __forceinline__ long long uniform64(long long x) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(unsigned long long)x);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)x >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ RaggedInstance uniform_instance(const RaggedInstance& v) {
+  RaggedInstance u;
+  u.n = __builtin_amdgcn_readfirstlane(v.n);
+  u.n_dense = __builtin_amdgcn_readfirstlane(v.n_dense);
+  u.heff = uniform64(v.heff);
+  u.dmat = uniform64(v.dmat);
+  u.lops = uniform64(v.lops);
+  u.rho = uniform64(v.rho);
+  u.work = uniform64(v.work);
+  return u;
+}
+
+template <int TILE, int SG, int MINW>
+__global__ void __launch_bounds__(256 * SG) __attribute__((amdgpu_waves_per_eu(MINW)))
+lindblad_ragged_persistent(RaggedArgs r) {
+  using C = float2;
+  __shared__ __align__(16) unsigned char smem[SG][smem_bytes<TILE, float, 0>()];
+  __shared__ int s_item;
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), ltid = threadIdx.x & 255;   // grp: wave-uniform
+  for (;;) {
+    __syncthreads();                          // s_item of the previous instance is consumed
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(r.queue, 1u);
+    __syncthreads();
+    // the item and the instance record are wave-uniform: kept in scalar registers, so the
+    // instance's pointers cost no vector registers in the tile code below
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= r.count) break;
+    const RaggedInstance in = uniform_instance(r.inst[__builtin_amdgcn_readfirstlane(r.order[item])]);
+    const int n = in.n;
+    const size_t nn = (size_t)n * n;
+    TiledArgs a = {};
+    a.batch = 1;
+    a.n = n;
+    a.nc = n;
+    a.group = 1;
+    a.n_dense = in.n_dense;
+    a.dt = r.dt;
+    a.rec = -1;
+    a.heff = r.ops + in.heff;
+    a.dmat = in.dmat >= 0 ? r.ops + in.dmat : nullptr;
+    a.lops = in.n_dense > 0 ? r.ops + in.lops : nullptr;
+    a.rho = r.rho + in.rho;
+    C* s0 = r.work + in.work;
+    C* s1 = s0 + nn;
+    a.inc = s1 + nn;
+    a.tbuf = s1 + 2 * nn;
+    for (size_t i = threadIdx.x; i < nn; i += blockDim.x) s0[i] = make_float2((float)a.rho[i].x, (float)a.rho[i].y);
+    __syncthreads();
+    const int tiles = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+    for (int step = 0; step < r.nsteps; ++step) {
+      for (int st = 0; st < 4; ++st) {
+        a.stage = st;
+        a.s_cur = (st % 2 == 0) ? s0 : s1;
+        a.s_next = (st % 2 == 0) ? s1 : s0;
+        // every group runs the same number of tiles (a group past the last tile runs a
+        // dummy one: all its MFMA blocks lie outside the operator and are skipped), so the
+        // panel barriers inside the GEMMs match across the workgroup
+        for (int k = 0; k < a.n_dense; ++k) {
+          for (int t0 = 0; t0 < tiles; t0 += SG) {
+            const int t = t0 + grp;
+            const Geo g = geometry<TILE>(n, t < tiles ? t : tiles, ltid);   // tiles: rows >= n
+            left_tile<TILE, float, 0>(a, 0, k, g, smem[grp]);
+          }
+        }
+        if (a.n_dense > 0) __syncthreads();   // T_k complete
+        constexpr int BW = TILE / 32;
+        for (int t0 = 0; t0 < tiles; t0 += SG) {
+          const int t = t0 + grp;
+          const Geo g = geometry<TILE>(n, t < tiles ? t : tiles, ltid);   // tiles: rows >= n
+          floatx4 cr[BW][BW], ci[BW][BW];
+#pragma unroll
+          for (int p = 0; p < BW; ++p)
+#pragma unroll
+            for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+          generator_gemms<TILE, float, 0>(a, 0, g, smem[grp], cr, ci);
+#pragma unroll
+          for (int p = 0; p < BW; ++p)
+#pragma unroll
+            for (int q = 0; q < BW; ++q)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<float>::row(g.lg, v);
+                const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+                if (i < n && j < n) stage_elem<float>(a, 0, 0, (size_t)i * n + j, cr[p][q][v], ci[p][q][v]);
+              }
+        }
+        __syncthreads();                        // the next stage operand is complete
+      }
+    }
+  }
+  if (threadIdx.x == 0 && atomicAdd(r.queue + 1, 1u) == gridDim.x - 1) {
+    // every other workgroup has left the loop: reset the queue for the next launch
+    atomicExch(r.queue, 0u);
+    atomicExch(r.queue + 1, 0u);
+  }
 }
 
 // Gradient contractions of an adjoint stage operator kappa with a forward stage operator
@@ -737,8 +861,6 @@ __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   extern __shared__ __attribute__((aligned(16))) unsigned char craw[];
   const int b = blockIdx.x, n = c.n;
   const size_t nn = (size_t)n * n;
-  const double2* K = c.kappa + (size_t)b * nn;
-  const double2* S = c.s + (size_t)b * nn;
   const int CQ = contract_bands(n);
   const int TQ = CT_THREADS / CQ;                    // threads per band (>= n when n <= 1024)
   double2* cs = reinterpret_cast<double2*>(craw);   // [CQ][n] column-sum partials
@@ -748,36 +870,47 @@ __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   double2* rp = rs + n;                              // row-pair terms
   const int q = threadIdx.x / TQ, tq = threadIdx.x % TQ;
   const int rows = (n + CQ - 1) / CQ, m0 = q * rows, m1 = q < CQ ? min(n, m0 + rows) : m0;
+  // every pair (kappa_p, s_p) adds into the same partial sums, pair by pair in a fixed order
   for (int j = tq; j < n && q < CQ; j += TQ) {
-    double sr = 0, si = 0, pr = 0, pi = 0;
-    for (int m = m0; m < m1; ++m) {
-      const double2 k = K[(size_t)m * n + j], x = S[(size_t)m * n + j];
-      sr += k.x * x.x + k.y * x.y;
-      si += k.x * x.y - k.y * x.x;
-      if (j + 1 < n) {
-        const double2 k1 = K[(size_t)m * n + j + 1], x1 = S[(size_t)m * n + j + 1];
-        pr += (k1.x * x.x + k1.y * x.y) + (k.x * x1.x + k.y * x1.y);
-        pi += (k1.x * x.y - k1.y * x.x) + (k.x * x1.y - k.y * x1.x);
+    double sr = 0, si = 0, pr = 0, pi = 0, dr = 0, di = 0;
+    for (int pp = 0; pp < c.pairs; ++pp) {
+      const double2* K = c.kappa[pp] + (size_t)b * nn;
+      const double2* S = c.s[pp] + (size_t)b * nn;
+      for (int m = m0; m < m1; ++m) {
+        const double2 k = K[(size_t)m * n + j], x = S[(size_t)m * n + j];
+        sr += k.x * x.x + k.y * x.y;
+        si += k.x * x.y - k.y * x.x;
+        if (j + 1 < n) {
+          const double2 k1 = K[(size_t)m * n + j + 1], x1 = S[(size_t)m * n + j + 1];
+          pr += (k1.x * x.x + k1.y * x.y) + (k.x * x1.x + k.y * x1.y);
+          pi += (k1.x * x.y - k1.y * x.x) + (k.x * x1.y - k.y * x1.x);
+        }
+      }
+      if (q == 0) {
+        const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
+        dr += kd.x * sd.x + kd.y * sd.y;
+        di += kd.x * sd.y - kd.y * sd.x;
       }
     }
     cs[q * n + j] = make_double2(sr, si);
     cp[q * n + j] = make_double2(pr, pi);
-    if (q == 0) {
-      const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
-      dg[j] = make_double2(kd.x * sd.x + kd.y * sd.y, kd.x * sd.y - kd.y * sd.x);
-    }
+    if (q == 0) dg[j] = make_double2(dr, di);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int r = wave; r < n; r += nw) {
     double sr = 0, si = 0, pr = 0, pi = 0;
-    for (int m = lane; m < n; m += 64) {
-      const double2 k = K[(size_t)r * n + m], x = S[(size_t)r * n + m];
-      sr += k.x * x.x + k.y * x.y;
-      si += k.x * x.y - k.y * x.x;
-      if (r + 1 < n) {
-        const double2 k1 = K[(size_t)(r + 1) * n + m], x1 = S[(size_t)(r + 1) * n + m];
-        pr += (k.x * x1.x + k.y * x1.y) + (k1.x * x.x + k1.y * x.y);
-        pi += (k.x * x1.y - k.y * x1.x) + (k1.x * x.y - k1.y * x.x);
+    for (int pp = 0; pp < c.pairs; ++pp) {
+      const double2* K = c.kappa[pp] + (size_t)b * nn;
+      const double2* S = c.s[pp] + (size_t)b * nn;
+      for (int m = lane; m < n; m += 64) {
+        const double2 k = K[(size_t)r * n + m], x = S[(size_t)r * n + m];
+        sr += k.x * x.x + k.y * x.y;
+        si += k.x * x.y - k.y * x.x;
+        if (r + 1 < n) {
+          const double2 k1 = K[(size_t)(r + 1) * n + m], x1 = S[(size_t)(r + 1) * n + m];
+          pr += (k.x * x1.x + k.y * x1.y) + (k1.x * x.x + k1.y * x.y);
+          pi += (k.x * x1.y - k.y * x1.x) + (k1.x * x.y - k1.y * x.x);
+        }
       }
     }
     sr = wave_sum(sr);
@@ -813,6 +946,39 @@ __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   }
 }
 
+// Start of a backward RK4 step: lam += g[b] m;  k4 = c lam;  k4_op = (operand precision) k4;
+// acc = lam.  One pass over the operators instead of four.
+template <typename C>
+__global__ void __launch_bounds__(256) adjoint_seed(SeedArgs s) {
+  const size_t nn = (size_t)s.n * s.n, total = (size_t)s.batch * nn;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = i / nn, o = i - b * nn;
+    const double2 g = s.g[b], m = s.m[o];
+    double2 l = s.lam[i];
+    l.x += g.x * m.x - g.y * m.y;
+    l.y += g.x * m.y + g.y * m.x;
+    s.lam[i] = l;
+    s.acc[i] = l;
+    const double2 k = make_double2(s.c * l.x, s.c * l.y);
+    s.k4[i] = k;
+    C kc;
+    kc.x = k.x;
+    kc.y = k.y;
+    static_cast<C*>(s.k4_op)[i] = kc;
+  }
+}
+
+hipError_t seed_launch(const SeedArgs& s, hipStream_t stream) {
+  const size_t total = (size_t)s.batch * s.n * s.n;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (s.op_fp64)
+    hipLaunchKernelGGL(adjoint_seed<double2>, dim3((unsigned)blocks), dim3(256), 0, stream, s);
+  else
+    hipLaunchKernelGGL(adjoint_seed<float2>, dim3((unsigned)blocks), dim3(256), 0, stream, s);
+  return hipGetLastError();
+}
+
 size_t contract_lds_bytes(int n) { return (size_t)(2 * contract_bands(n) + 3) * n * sizeof(double2); }
 
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
@@ -827,8 +993,9 @@ hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
 // Bordered layout when n = nc + 1 with nc a multiple of the tile and no dense operators
 // (their products T_k = L_k S would need a border of their own); sets a.nc, a.border_wgs.
 // Returns the workgroups per operator of the stage / apply kernels.
+// (The border dot products carry both H_eff products, so one-sided generators never use it.)
 static int layout(TiledArgs& a, int tile) {
-  const bool brd = a.n_dense == 0 && a.n > tile && (a.n - 1) % tile == 0;
+  const bool brd = a.n_dense == 0 && !(a.flags & kFlagOneSided) && a.n > tile && (a.n - 1) % tile == 0;
   a.nc = brd ? a.n - 1 : a.n;
   a.border_wgs = brd ? (2 * a.nc + 1 + BG_OUT - 1) / BG_OUT : 0;
   const int tpd = (a.nc + tile - 1) / tile;
@@ -944,14 +1111,16 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workgroups per operator of the stage kernel for a tile edge (the bordered layout adds
 // the border workgroups).
-static int stage_wgs(int n, int n_dense, int tile) {
+static int stage_wgs(int n, int n_dense, int flags, int tile) {
   TiledArgs t = {};
   t.n = n;
   t.n_dense = n_dense;
+  t.flags = flags;
   return layout(t, tile);
 }
 
-static WorkLayout work_layout(int batch, int n, int n_dense, bool fp64, bool renorm, int n_eops) {
+static WorkLayout work_layout(int batch, int n, int n_dense, int flags, bool fp64, int n_eops) {
+  const bool renorm = (flags & 1) != 0;
   const size_t op = (size_t)batch * n * n * (fp64 ? sizeof(double2) : sizeof(float2));
   WorkLayout w;
   w.s0 = 0;
@@ -961,14 +1130,14 @@ static WorkLayout work_layout(int batch, int n, int n_dense, bool fp64, bool ren
   w.tscale = w.tbuf + align256(op * (size_t)n_dense);
   w.obs_part = w.tscale + (renorm ? align256((size_t)batch * sizeof(double)) : 0);
   // slots for the larger of the two tile choices (32-tiles: more workgroups)
-  const int slots = stage_wgs(n, n_dense, 32) > stage_wgs(n, n_dense, 64) ? stage_wgs(n, n_dense, 32)
-                                                                          : stage_wgs(n, n_dense, 64);
+  const int w32 = stage_wgs(n, n_dense, flags, 32), w64 = stage_wgs(n, n_dense, flags, 64);
+  const int slots = w32 > w64 ? w32 : w64;
   w.total = w.obs_part + (n_eops > 0 ? align256((size_t)batch * slots * n_eops * sizeof(double2)) : 0);
   return w;
 }
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm, int n_eops) {
-  return work_layout(batch, n, n_dense, fp64, renorm, n_eops).total;
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, int flags, bool fp64, int n_eops) {
+  return work_layout(batch, n, n_dense, flags, fp64, n_eops).total;
 }
 
 int tiled_pick(int batch, int n) {
@@ -981,8 +1150,7 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
                           int tile) {
   using C = typename CT<R>::C;
   const size_t nn = (size_t)a.n * a.n;
-  const WorkLayout wl = work_layout(a.batch, a.n, a.n_dense, sizeof(R) == 8, (a.flags & 1) != 0,
-                                    a.obs ? a.n_eops : 0);
+  const WorkLayout wl = work_layout(a.batch, a.n, a.n_dense, a.flags, sizeof(R) == 8, a.obs ? a.n_eops : 0);
   unsigned char* base = static_cast<unsigned char*>(work);
   C* s0 = reinterpret_cast<C*>(base + wl.s0);
   C* s1 = reinterpret_cast<C*>(base + wl.s1);
@@ -990,7 +1158,7 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
   a.tbuf = base + wl.tbuf;
   a.tscale = reinterpret_cast<double*>(base + wl.tscale);
   a.obs_part = reinterpret_cast<double2*>(base + wl.obs_part);
-  a.slots = stage_wgs(a.n, a.n_dense, tile);
+  a.slots = stage_wgs(a.n, a.n_dense, a.flags, tile);
   const size_t cnt = (size_t)a.batch * nn;
   hipLaunchKernelGGL(init_stage_operand<C>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, a.rho,
                      s0, cnt);
@@ -1011,6 +1179,44 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
     }
   }
   return hipGetLastError();
+}
+
+
+// ---- persistent ragged launch: shapes (TILE, SG, MINW) by variant; MINW is the waves per
+// SIMD the register budget is compiled for (512 / MINW VGPRs)
+namespace {
+constexpr int kRaggedVariants = 5;
+
+template <int TILE, int SG, int MINW>
+hipError_t ragged_launch_shape(const RaggedArgs& r, int max_groups, hipStream_t stream) {
+  auto k = lindblad_ragged_persistent<TILE, SG, MINW>;
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), 256 * SG, 0);
+  if (e != hipSuccess) return e;
+  int groups = cus * (per_cu > 0 ? per_cu : 1);
+  if (max_groups > 0 && groups > max_groups) groups = max_groups;
+  if (groups > r.count) groups = r.count;
+  hipLaunchKernelGGL(k, dim3(groups), dim3(256 * SG), 0, stream, r);
+  return hipGetLastError();
+}
+}  // namespace
+
+int ragged_variants() { return kRaggedVariants; }
+
+hipError_t ragged_run(const RaggedArgs& r, int variant, int max_groups, hipStream_t stream) {
+  if (r.count <= 0 || r.nsteps <= 0) return hipSuccess;
+  switch (variant) {
+    case 0: return ragged_launch_shape<64, 2, 2>(r, max_groups, stream);   // 8 waves, 1 per CU
+    case 1: return ragged_launch_shape<64, 1, 2>(r, max_groups, stream);   // 4 waves, 2 per CU
+    case 2: return ragged_launch_shape<32, 4, 4>(r, max_groups, stream);   // 16 waves
+    case 3: return ragged_launch_shape<32, 2, 4>(r, max_groups, stream);   // 8 waves, 2 per CU
+    case 4: return ragged_launch_shape<32, 1, 4>(r, max_groups, stream);   // 4 waves, 4 per CU
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,

@@ -46,22 +46,60 @@ enum class TiledPrec { FP32 = 0, SPLIT_BF16 = 1, FP64 = 2 };
 // Workspace of lime_lindblad_rk4's per-stage path: S0, S1, the increment, T_k (operand
 // precision), the trace scales (renormalisation) and the per-workgroup observable partials
 // (n_eops > 0), each region 256-byte aligned.
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, bool fp64, bool renorm, int n_eops);
+size_t tiled_workspace_bytes(int batch, int n, int n_dense, int flags, bool fp64, int n_eops);
+
+// lime_lindblad_batch.flags bit of one-sided generators (LIME_F_ONE_SIDED)
+constexpr int kFlagOneSided = 0x200;
 int tiled_pick(int batch, int n);
 hipError_t tiled_run(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
                      int tile, TiledPrec prec);
 hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64);
 
+constexpr int kMaxContractPairs = 4;
 struct ContractArgs {
-  int batch, n;
-  const double2* kappa;   // adjoint stage operator  [batch][n][n]
-  const double2* s;       // forward stage operator  [batch][n][n]
+  int batch, n, pairs;    // pairs (<= kMaxContractPairs) of operators summed into one result
+  const double2* kappa[kMaxContractPairs];   // adjoint stage operators  [batch][n][n]
+  const double2* s[kMaxContractPairs];       // forward stage operators  [batch][n][n]
   double weight;
   double2* out_a;         // [batch][n]   += w (rowsum - colsum) of conj(kappa) o s
   double2* out_b;         // [batch][n]   += w (diag - (rowsum + colsum)/2)
   double2* out_u;         // [batch][n-1] += w (nearest-neighbour coupling contraction)
 };
 size_t contract_lds_bytes(int n);
+
+// Start of one backward RK4 step of the adjoint (elementwise over batch n^2):
+//   lam += g[b] m;  k4 = c lam;  k4_op = k4 in the operand precision;  acc = lam
+struct SeedArgs {
+  int batch, n, op_fp64;
+  double2* lam;           // [batch][n][n]
+  const double2* g;       // [batch] source weights
+  const double2* m;       // [n][n] source operator (shared)
+  double c;
+  double2* k4;            // [batch][n][n]
+  void* k4_op;            // [batch][n][n] complex64 (complex128 when op_fp64)
+  double2* acc;           // [batch][n][n]
+};
+hipError_t seed_launch(const SeedArgs& s, hipStream_t stream);
+
+// Uneven batch in one persistent launch (complex64 operands on FP32 MFMA, FP64 state).
+struct RaggedInstance {
+  int n, n_dense;
+  long long heff, dmat, lops;   // element offsets into ops (complex64); dmat < 0: none
+  long long rho;                // element offset into rho (complex128)
+  long long work;               // element offset into work (complex64): (3 + n_dense) n^2
+};
+struct RaggedArgs {
+  const RaggedInstance* inst;   // [count]
+  const int* order;             // [count] processing order (largest cost first)
+  int count, nsteps;
+  double dt;
+  const float2* ops;
+  double2* rho;
+  float2* work;
+  unsigned* queue;              // [2]: next item, finished workgroups (zero between launches)
+};
+int ragged_variants();
+hipError_t ragged_run(const RaggedArgs& r, int variant, int max_groups, hipStream_t stream);
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream);
 
 }  // namespace lime

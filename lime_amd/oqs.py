@@ -6,7 +6,8 @@ Every propagation below runs the fused HIP kernel ``lime_lindblad_rk4`` through
 Reference functions and the decisions taken where lime's behaviour is broken:
 
 * ``_lindblad``            -- oqs.py:1590-1688.  Copies rho0 (never mutated), complex128
-  state, records observables/states *after* each step (t_k = t0 + (k+1) dt).  lime's
+  state, records observables/states *after* each step (t_k = t0 + (k+1) dt; the text
+  files carry lime's accumulated ``t += dt`` times, ``lime_amd.datfile``).  lime's
   ``Result`` crash (mol.py:92) is fixed (``lime_amd.mol.Result``).  With
   ``return_result=False`` lime writes observables taken *before* the step under the time
   *after* it (oqs.py:1638-1658); here ``obs.dat`` holds the after-step values.
@@ -28,11 +29,12 @@ Extensions (lime has no batching): ``H`` may be (B, N, N) and ``rho0`` (B, N, N)
 import numpy as np
 import torch
 
+from . import datfile
 from . import phys
 from . import superoperator as superop
 from .mol import Result
 from .propagator import LindbladPropagator, as_c128, device
-from .redfield import Redfield_solver, _redfield, redfield_tensor  # noqa: F401  (lime.oqs names)
+from .redfield import Redfield_solver, _redfield, getG, redfield_tensor  # noqa: F401  (lime.oqs names)
 
 
 def _np(t):
@@ -102,12 +104,9 @@ def _lindblad(H, rho0, c_ops, e_ops=None, Nt=1, dt=0.005, return_result=True, t0
     nrec = Nt // nout
     obs_np = _np(obs) if obs is not None else np.zeros((rho.shape[0], nrec, 0), np.complex128)
     if not return_result:
-        with open("obs.dat", "w") as f:
-            fmt = "{} " * (len(e_ops) + 1) + "\n"
-            for r in range(nrec):
-                vals = obs_np[0, r] if not batched else obs_np[:, r].ravel()
-                f.write(fmt.format(t0 + (r + 1) * nout * dt, *vals) if not batched
-                        else " ".join(str(x) for x in (t0 + (r + 1) * nout * dt, *vals)) + "\n")
+        t = datfile.time_column(dt, Nt, t0=t0, every=nout)[:nrec]
+        rows = obs_np[0] if not batched else obs_np.transpose(1, 0, 2).reshape(nrec, -1)
+        datfile.write_obs("obs.dat", t, rows)
         out = _np(rho)
         return out if batched else out[0]
     result = Result(dt=dt, Nt=Nt, rho0=rho0, t0=t0, nout=nout)
@@ -128,9 +127,11 @@ def _correlation_2p_1t(H, rho0, ops, c_ops, dt, Nt, method="lindblad", output="c
     _, obs, _ = _run(H, x0, c_ops, dt, Nt, [A])
     cor = _np(obs)[0, :, 0] if obs.shape[0] == 1 else _np(obs)[:, :, 0]
     if output:
-        with open(output, "w") as f:
-            for k in range(Nt):
-                f.write("{} {} \n".format((k + 1) * dt, cor[k] if cor.ndim == 1 else cor[:, k]))
+        t = datfile.time_column(dt, Nt)
+        if cor.ndim == 1:
+            datfile.write_cor(output, t, cor)
+        else:                              # a batch: one column per operator
+            datfile.write_obs(output, t, cor.T)
     return cor
 
 
@@ -222,3 +223,65 @@ class Lindblad_solver:
             raise ValueError("Number of operators is not 4.")
         a, b, c, d = ops
         return self.correlation_3op_2t(rho0, [a, b @ c, d], dt, nt, ntau)
+
+
+class HEOMSolverDL(Lindblad_solver):
+    """oqs.py:1335-1431.  Despite its name, lime's class propagates the Lindblad master
+    equation (``solve`` -> ``_lindblad``, ``correlation_2op_1t`` -> ``_correlation_2p_1t``,
+    ``correlation_3op_2t`` the same two-time driver as ``Lindblad_solver``); it runs on the
+    same fused kernel here."""
+
+    def solve(self, rho0, dt, Nt, return_result=True):
+        return _lindblad(self.H, rho0, self.c_ops, e_ops=self.e_ops, Nt=Nt, dt=dt,
+                         return_result=return_result, precision=self.precision)
+
+
+class OQS:
+    """oqs.py:583-697 (open-quantum-system container).  lime's ``set_hamiltonian``/``setH``
+    assign an undefined name and ``correlation_2p_1t`` calls an undefined module function
+    (oqs.py:601-607, :694); here they set the Hamiltonian and run ``_correlation_2p_1t``
+    (QRT <A(t)B> on the fused kernel, ``output`` written in lime's cor.dat format)."""
+
+    def __init__(self, H, c_ops=None):
+        self.hamiltonian = H
+        self.h_sys = H
+        self.H = H
+        self.nstates = np.shape(H)[-1]
+        self.e_ops = None
+        self.c_ops = c_ops
+
+    def set_hamiltonian(self, h):
+        self.hamiltonian = self.h_sys = self.H = h
+        self.nstates = np.shape(h)[-1]
+
+    setH = set_hamiltonian
+
+    def set_c_ops(self, c_ops):
+        self.c_ops = c_ops
+
+    def set_e_ops(self, e_ops):
+        self.e_ops = e_ops
+
+    def configure(self, c_ops, e_ops):
+        self.c_ops = c_ops
+        self.e_ops = e_ops
+
+    def correlation_2p_1t(self, rho0, ops, dt, Nt, method="lindblad", output="cor.dat"):
+        return _correlation_2p_1t(self.hamiltonian, rho0, ops=ops, c_ops=self.c_ops or [],
+                                  dt=dt, Nt=Nt, method=method, output=output)
+
+
+# ------------------------------------------------------------------ state preparation
+def coherent(N, alpha):
+    """Coherent state D(alpha)|0> in an N-level truncated Fock space (oqs.py:856-922): the
+    displacement operator exp(alpha a^dag - conj(alpha) a) of the truncated ladder operator
+    applied to the vacuum, so the state is normalised.  Returns a 1-D complex128 array."""
+    from scipy.linalg import expm
+    a = phys.destroy(N)
+    D = expm(alpha * a.conj().T - np.conj(alpha) * a)
+    return D @ phys.basis(N, 0)
+
+
+def coherent_dm(N, alpha):
+    """|alpha><alpha| of ``coherent(N, alpha)`` (oqs.py:926-981)."""
+    return phys.ket2dm(coherent(N, alpha))

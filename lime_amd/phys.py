@@ -8,6 +8,8 @@ The propagation itself never goes through ``rk4`` here: the solvers call the fus
 * ``obs_dm``     -- phys.py:837-844: Tr[d rho] (no conjugate on d), here in O(N^2)
 * ``transform``  -- phys.py:706-718: v^dag A v
 * ``isherm``, ``ket2dm``, ``basis``, ``destroy``, ``pauli`` -- phys.py:1429, 579, 879, 615, 773
+* ``expm``       -- phys.py:1352-1401 ('EOM': U(t) from dU/dt = A U by RK4, on the GPU
+  kernels); ``ldo`` -- phys.py:1409-1425 (A b)
 """
 import numpy as np
 
@@ -90,3 +92,41 @@ def pauli():
     sy = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)
     sz = np.array([[1, 0], [0, -1]], dtype=np.complex128)
     return s0, sx, sy, sz
+
+
+def ldo(b, A):
+    """Linear differential operator A b (phys.py:1409-1425)."""
+    return A @ b
+
+
+def expm(A, t, method="EOM", precision="fp64"):
+    """U(t_k) = e^{A t_k} for the times t (phys.py:1352-1401), as lime's list of Nt arrays.
+
+    'EOM': U(t_0) = I and each further entry one RK4 step of dU/dt = A U with
+    dt = t[1] - t[0] (lime's ``interval``), i.e. U(t_k) = P(dt A)^k with the RK4 polynomial
+    P; the Nt - 1 steps run in one launch sequence of the one-sided per-stage kernels
+    (FP64 by default) recording every step.  'SOS' raises NotImplementedError, as in lime.
+    """
+    if method == "SOS":
+        raise NotImplementedError("Method of SOS has not been implemented. Choose from EOM")
+    if method != "EOM":
+        raise ValueError(f"unknown method {method!r}")
+    import torch
+
+    from .redfield import _column_chunk, liouville_propagator
+    t = np.asarray(t, dtype=np.float64)
+    Nt = len(t)
+    prop = liouville_propagator(A, precision)
+    M = prop.n
+    U = torch.eye(M, dtype=torch.complex128, device=prop.dev).unsqueeze(0).contiguous()
+    out = [U[0].cpu().numpy()]
+    if Nt > 1:
+        dt = float(t[1] - t[0])
+        done, chunk = 0, _column_chunk(M)
+        while done < Nt - 1:
+            m = min(chunk, Nt - 1 - done)
+            st = torch.empty((1, m, M, M), dtype=torch.complex128, device=prop.dev)
+            prop.step(U, dt, m, states=st, nout=1, step_offset=0, rec_capacity=m)
+            out.extend(st[0].cpu().numpy())
+            done += m
+    return out

@@ -116,6 +116,7 @@ class LindbladPropagator:
         kernels even for n <= 96."""
         self.dev = device()
         self.force_tiled = bool(force_tiled)
+        self.one_sided = False
         self.variant = int(variant)
         self._work = None
         if precision not in _PREC_FLAGS:
@@ -148,12 +149,17 @@ class LindbladPropagator:
 
     @classmethod
     def from_folded(cls, heff, dmat=None, lops=None, rops=None, group=1, precision="fp32",
-                    force_tiled=False):
-        """Build directly from H_eff (S,N,N), D (S,N,N), dense L / R (S,K,N,N) tensors."""
+                    force_tiled=False, one_sided=False):
+        """Build directly from H_eff (S,N,N), D (S,N,N), dense L / R (S,K,N,N) tensors.
+
+        ``one_sided``: drop the X H_eff^dag product, f(X) = -i H_eff X + D o X + sum L X R^dag
+        -- with H_eff = i G this steps dX/dt = G X (a Liouville-space generator acting on
+        the columns of X; per-stage kernels)."""
         self = object.__new__(cls)
         self.dev = device()
         self.precision = precision
-        self.force_tiled = bool(force_tiled)
+        self.one_sided = bool(one_sided)
+        self.force_tiled = bool(force_tiled) or self.one_sided
         self.variant = 0
         self._work = None
         self._lib = _lib.load()
@@ -195,7 +201,8 @@ class LindbladPropagator:
     def _resident(self):
         """Does ``step`` run the resident kernel (whose ``variant`` is a launch shape, not
         the tile edge of the per-stage kernels)?"""
-        return not self.force_tiled and self.precision == "fp32" and self.n <= self.max_n()
+        return (not self.force_tiled and not self.one_sided and self.precision == "fp32"
+                and self.n <= self.max_n())
 
     def _tile_variant(self):
         """The ``variant`` field for the per-stage kernels (``apply`` always runs them): a
@@ -228,11 +235,14 @@ class LindbladPropagator:
         adj.rops = h(self.rops) if self.rops is not None else None
         return adj
 
+    def _flags(self):
+        return _PREC_FLAGS[self.precision] | (_lib.LIME_F_ONE_SIDED if self.one_sided else 0)
+
     def _descriptor(self, B):
         d = _lib.LindbladBatch()
         d.batch, d.n, d.group, d.n_dense = B, self.n, self.group, self.n_dense
         d.eop_group = B
-        d.flags = _PREC_FLAGS[self.precision]
+        d.flags = self._flags()
         d.variant = self.variant
         d.heff = self.heff.data_ptr()
         d.dmat = self.dmat.data_ptr() if self.dmat is not None else None
@@ -310,7 +320,7 @@ class LindbladPropagator:
         d.batch, d.n, d.group, d.n_dense = B, n, self.group, self.n_dense
         d.n_eops = n_e
         d.eop_group = eop_group
-        d.flags = _PREC_FLAGS[self.precision] | (_lib.LIME_F_RENORM_TRACE if renorm else 0) \
+        d.flags = self._flags() | (_lib.LIME_F_RENORM_TRACE if renorm else 0) \
             | (_lib.LIME_F_FORCE_TILED if self.force_tiled else 0)
         d.variant = self.variant
         d.heff = self.heff.data_ptr()

@@ -76,37 +76,92 @@ def redfield_tensor(H, a_ops, spectra, secular=False):
     return sp.csr_matrix(R), evecs
 
 
-def _redfield(R, rho0, evecs=None, Nt=1, dt=0.005, t0=0, e_ops=None, return_result=True):
-    """RK4 on vec rho with dense R on the GPU (oqs.py:373-468 semantics)."""
+def liouville_propagator(G, precision="fp64"):
+    """Kernel stepping dX/dt = G X for a dense Liouville-space generator G (M x M): the
+    one-sided generator of the per-stage kernels with H_eff = i G.  FP64 by default (these
+    are lime's complex128 Liouville-space paths)."""
+    dev = device()
+    Gt = as_c128(_dense(G), dev)
+    return LindbladPropagator.from_folded(1j * Gt, one_sided=True, precision=precision)
+
+
+def _column_chunk(M, budget=1 << 28):
+    """RK4 steps per launch when every step's M x M state is recorded (<= 256 MB)."""
+    return max(1, budget // (M * M * 16))
+
+
+def _redfield(R, rho0, evecs=None, Nt=1, dt=0.005, t0=0, e_ops=None, return_result=True,
+              precision="fp64"):
+    """RK4 on vec rho with a dense Liouville tensor R (oqs.py:373-468 semantics).
+
+    vec rho is column 0 of an M x M operator X (M = N^2, other columns zero), stepped by
+    the one-sided kernels (dX/dt = R X) in launches of many steps; Tr[e rho] is the kernel's
+    record sum_m W_m0 X_m0 with W_m0 = e[m % N, m // N].  States are transformed back with
+    evecs (rho -> v rho v^dag), observables are taken in the eigenbasis, as lime does."""
     dev = device()
     e_ops = [] if e_ops is None else list(e_ops)
-    n = _dense(rho0).shape[0]
+    r0 = _dense(rho0)
+    n = r0.shape[0]
+    M = n * n
     if evecs is not None:
-        rho0 = evecs.conj().T @ _dense(rho0) @ evecs
+        r0 = evecs.conj().T @ r0 @ evecs
         e_ops = [evecs.conj().T @ _dense(e) @ evecs for e in e_ops]
-    Rm = as_c128(_dense(R), dev)
-    v = as_c128(_dense(rho0).flatten(), dev)
-    E = torch.stack([as_c128(e, dev).transpose(0, 1).contiguous().flatten() for e in e_ops]) if e_ops else None
-    V = as_c128(evecs, dev) if evecs is not None else None
-    obs = torch.zeros((Nt, len(e_ops)), dtype=torch.complex128, device=dev)
-    states = []
-    for k in range(Nt):
-        k1 = Rm @ v
-        k2 = Rm @ (v + 0.5 * dt * k1)
-        k3 = Rm @ (v + 0.5 * dt * k2)
-        k4 = Rm @ (v + dt * k3)
-        v = v + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
-        if E is not None:
-            obs[k] = E @ v
-        if return_result:
-            tmp = v.reshape(n, n)
-            states.append(V @ tmp @ V.conj().transpose(0, 1) if V is not None else tmp.clone())
+    prop = liouville_propagator(R, precision)
+    X = torch.zeros((1, M, M), dtype=torch.complex128, device=dev)
+    X[0, :, 0] = as_c128(r0.flatten(), dev)
+    Et = None
+    if e_ops:
+        Et = torch.zeros((1, len(e_ops), M, M), dtype=torch.complex128, device=dev)
+        for i, e in enumerate(e_ops):
+            Et[0, i, :, 0] = as_c128(_dense(e).T.flatten(), dev)     # e[m % N, m // N]
+    obs = torch.zeros((1, Nt, len(e_ops)), dtype=torch.complex128, device=dev) if e_ops else None
+    cols = torch.empty((Nt, M), dtype=torch.complex128, device=dev) if return_result else None
+    chunk = _column_chunk(M) if return_result else Nt
+    done = 0
+    while done < Nt:
+        m = min(chunk, Nt - done)
+        # records at chunk-local indices 0..m-1 (one launch sequence per chunk)
+        st = torch.empty((1, m, M, M), dtype=torch.complex128, device=dev) if return_result else None
+        ob = torch.empty((1, m, len(e_ops)), dtype=torch.complex128, device=dev) if e_ops else None
+        prop.step(X, dt, m, eops_t=Et, obs=ob, states=st, nout=1, step_offset=0, rec_capacity=m)
+        if ob is not None:
+            obs[:, done:done + m] = ob
+        if st is not None:
+            cols[done:done + m] = st[0, :, :, 0]
+        done += m
     if not return_result:
-        return _dense(v.reshape(n, n))
+        return _dense(X[0, :, 0].reshape(n, n))
     res = Result(dt=dt, Nt=Nt, rho0=rho0, t0=t0)
-    res.observables = _dense(obs)
-    res.rholist = [_dense(s) for s in states]
+    res.observables = _dense(obs[0]) if obs is not None else np.zeros((Nt, 0), np.complex128)
+    states = cols.reshape(Nt, n, n)
+    if evecs is not None:
+        V = as_c128(evecs, dev)
+        states = torch.matmul(torch.matmul(V, states), V.conj().transpose(0, 1))
+    res.rholist = list(_dense(states))
     return res
+
+
+def getG(L, t, w=None, k=6, domain="time"):
+    """Green's function of (i d/dt - L) by eigendecomposition (oqs.py:474-526), on the GPU.
+
+    time:  G(t) = -i U1 diag(exp(-i lambda t)) U1^-1        -> (M, M, Nt)
+    freq:  G(w) = U1 diag(1 / (w - lambda)) U1^-1 = (w - L)^-1 -> (M, M, Nw)
+    lime's frequency branch contracts with conj(U1^-1)[b, n] instead of U1^-1[n, b], which is
+    the resolvent only when U1 is symmetric; here it is the resolvent (w - L)^-1 itself."""
+    dev = device()
+    Lt = as_c128(_dense(L), dev)
+    lam, U1 = torch.linalg.eig(Lt)
+    U2 = torch.linalg.inv(U1)
+    if domain == "time":
+        tt = torch.as_tensor(np.asarray(t, dtype=np.float64), device=dev).to(torch.complex128)
+        ph = -1j * torch.exp(-1j * lam[:, None] * tt[None, :])              # (M, Nt)
+    elif domain == "freq":
+        ww = torch.as_tensor(np.asarray(w, dtype=np.complex128), device=dev)
+        ph = 1.0 / (ww[None, :] - lam[:, None])                              # (M, Nw)
+    else:
+        raise ValueError(f"unknown domain {domain!r}; expected 'time' or 'freq'")
+    G = torch.matmul(U1.unsqueeze(0) * ph.t().unsqueeze(1), U2.unsqueeze(0))  # (Nt, M, M)
+    return _dense(G.permute(1, 2, 0))
 
 
 class Redfield_solver:
@@ -176,17 +231,8 @@ class Redfield_solver:
         Rm = as_c128(_dense(self.R), dev)
         t = np.asarray(t, dtype=np.float64)
         if method == "EOM":
-            dt = t[1] - t[0]
-            U = torch.eye(Rm.shape[0], dtype=torch.complex128, device=dev)
-            Ul = []
-            for _ in range(len(t)):
-                Ul.append(U.clone())
-                k1 = Rm @ U
-                k2 = Rm @ (U + 0.5 * dt * k1)
-                k3 = Rm @ (U + 0.5 * dt * k2)
-                k4 = Rm @ (U + dt * k3)
-                U = U + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
-            Ut = torch.stack(Ul, dim=-1)
+            from .phys import expm
+            Ut = torch.stack([as_c128(u, dev) for u in expm(Rm, t)], dim=-1)
         elif method in ("eseries", "SOS"):
             w, U1 = torch.linalg.eig(Rm)
             U2 = torch.linalg.inv(U1)
@@ -199,9 +245,18 @@ class Redfield_solver:
         return self.U
 
     def gf(self, t, w=None, secular=False, k=1, domain="time", method="EOM"):
+        """Liouville-space Green's function G(t) = -i U(t) as (N^2, N^2, Nt) (oqs.py:145-166):
+        'EOM' steps dU/dt = R U with the one-sided kernels; 'eseries' / 'diag' /
+        'diagonalization' is getG(i R, t) = -i exp(R t) by eigendecomposition.  lime's EOM
+        branch fails (``-1j * list`` of ``expm``'s result, and R unbound when the tensor was
+        already computed); here both branches return the same array."""
         if self.R is None:
             self.redfield_tensor(secular=secular)
-        return -1j * self.propagator(t, method="EOM" if method == "EOM" else "SOS")
+        if method == "EOM":
+            return -1j * self.propagator(t, method="EOM")
+        if method in ("eseries", "diag", "diagonalization"):
+            return getG(1j * _dense(self.R), t)
+        raise ValueError(f"unknown method {method!r}")
 
     def expect(self, rho0, e_ops):
         """<<e|U(t)|rho0>> in the eigenbasis for each e_op (oqs.py:225-252), on the GPU."""

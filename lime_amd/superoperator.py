@@ -134,12 +134,24 @@ class Lindblad_solver:
         return self.L
 
     def eigenstates(self, k=None):
+        """Eigenpairs of the Liouvillian (superoperator.py:490-523) by a dense GPU
+        eigendecomposition; left eigenvectors are the rows of inv(V) (biorthonormal, complex
+        norms).  With ``k`` the k eigenvalues of largest real part (lime's ``eigs(.., k,
+        which='LR')``) are kept, ordered as lime's ``sort`` orders them (ascending by real,
+        then imaginary part), and every later call of this solver uses those k modes; lime's
+        own k branch returns unbound names.  k >= dim - 1 raises ValueError as in lime."""
         if k is not None and k >= self.dim - 1:
             raise ValueError("k should be < the size of the matrix.")
         dev = device()
         Ld = liouvillian_dense(self.H, self.c_ops, dev)
         w, vr = torch.linalg.eig(Ld)
         vl_h = torch.linalg.inv(vr)                      # rows: left eigenvectors^dag, vl^dag vr = I
+        if k is not None:
+            wn = _dense(w)
+            keep = np.argsort(-wn.real, kind="stable")[:k]
+            keep = keep[np.lexsort((wn[keep].imag, wn[keep].real))]
+            idx = torch.as_tensor(keep, device=dev)
+            w, vr, vl_h = w[idx], vr[:, idx], vl_h[idx, :]
         self.eigvals = w
         self.right_eigvecs = vr
         self.left_eigvecs = vl_h.conj().transpose(0, 1)
@@ -163,7 +175,7 @@ class Lindblad_solver:
         dev = self._ensure()
         n = int(round(math.sqrt(self.dim)))
         A = as_c128(_dense(a), dev)
-        U = self.right_eigvecs.transpose(0, 1).reshape(self.dim, n, n)     # U_n as matrices
+        U = self.right_eigvecs.transpose(0, 1).reshape(-1, n, n)           # U_n as matrices (kept modes)
         return torch.einsum("ij,mji->m", A, U)
 
     def evolve(self, rho0, tlist, e_ops):
@@ -217,9 +229,9 @@ class Lindblad_solver:
         dev = self._ensure()
         n = int(round(math.sqrt(self.dim)))
         wb = self._observe_row(b)                                            # (m,)
-        U = self.right_eigvecs.transpose(0, 1).reshape(self.dim, n, n)       # (n_vec, N, N)
+        U = self.right_eigvecs.transpose(0, 1).reshape(-1, n, n)             # (n_vec, N, N)
         Aop, Cop = as_c128(_dense(a), dev), as_c128(_dense(c), dev)
-        CUA = torch.matmul(torch.matmul(Cop, U), Aop).reshape(self.dim, self.dim).transpose(0, 1)
+        CUA = torch.matmul(torch.matmul(Cop, U), Aop).reshape(U.shape[0], self.dim).transpose(0, 1)
         mid = (self.left_eigvecs.conj().transpose(0, 1) @ CUA) / self.norm[:, None]
         c0 = self._coeff(rho0)
         coeff = wb[:, None] * mid * c0[None, :]

@@ -12,7 +12,11 @@ Source data (read as text, never executed): ``examples/cor.dat`` and
 * ``tlist = linspace(0, 10, 200)`` so dt = 10/199 and Nt = 200;
 * ``ops = [a^dag, num, a]``: the propagated operator starts at ``a rho0 a^dag`` and
   after every RK4 step ``cor = Tr[num rho]`` and ``dm = ravel(rho)`` (row-major) are written,
-  each line tagged with t = (k + 1) dt.
+  each line tagged with t, accumulated as ``t += dt`` from 0 (so the k-th line carries the
+  floating-point sum of k + 1 copies of dt, not (k + 1) * dt).
+
+The time column of both files is kept verbatim as text (``t_cor_text``, ``t_dm_text``) so
+that writers can be checked for byte-compatible output.
 
 Usage (in a container that holds the reference)::
 
@@ -28,35 +32,38 @@ def _parse_complex(tok: str) -> complex:
 
 
 def read_cor(path):
-    t, cor = [], []
+    t, cor, text = [], [], []
     with open(path) as f:
         for line in f:
             parts = line.split()
             if not parts:
                 continue
+            text.append(parts[0])
             t.append(float(parts[0]))
             cor.append(_parse_complex(parts[1]))
-    return np.array(t), np.array(cor, dtype=np.complex128)
+    return np.array(t), np.array(cor, dtype=np.complex128), np.array(text)
 
 
 def read_dm(path):
-    t, rows = [], []
+    t, rows, text = [], [], []
     with open(path) as f:
         for line in f:
             parts = line.split()
             if not parts:
                 continue
+            text.append(parts[0])
             t.append(float(parts[0]))
             rows.append([_parse_complex(p) for p in parts[1:]])
-    return np.array(t), np.array(rows, dtype=np.complex128)
+    return np.array(t), np.array(rows, dtype=np.complex128), np.array(text)
 
 
 def main(src_dir, out_path):
-    t_cor, cor = read_cor(f"{src_dir}/cor.dat")
-    t_dm, dm = read_dm(f"{src_dir}/dm.dat")
+    t_cor, cor, t_cor_text = read_cor(f"{src_dir}/cor.dat")
+    t_dm, dm, t_dm_text = read_dm(f"{src_dir}/dm.dat")
     n = int(round(np.sqrt(dm.shape[1])))
     np.savez_compressed(out_path, t_cor=t_cor, cor=cor, t_dm=t_dm,
                         dm=dm.reshape(len(t_dm), n, n),
+                        t_cor_text=t_cor_text, t_dm_text=t_dm_text,
                         kappa=0.25, n_th=2.0, nlevels=n, t_final=10.0, npts=200)
     print(f"wrote {out_path}: cor {cor.shape}, dm {dm.shape} -> ({len(t_dm)}, {n}, {n})")
 

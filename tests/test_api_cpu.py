@@ -58,3 +58,24 @@ def test_no_cpu_fallback_without_gpu():
     from lime_amd.oqs import Lindblad_solver
     with pytest.raises(RuntimeError, match="GPU"):
         Lindblad_solver(np.eye(2), []).evolve(np.eye(2) / 2, 0.1, 2)
+
+
+def test_coherent_states_against_lime_docstring_values():
+    """lime's oqs.py:885-892 and :951-957 quote coherent(5, 0.25j) and coherent_dm(3, 0.25j);
+    the untruncated Fock coefficients e^{-|a|^2/2} a^n / sqrt(n!) fix a larger space."""
+    from math import factorial
+
+    from lime_amd.oqs import coherent, coherent_dm
+    psi = coherent(5, 0.25j)
+    quoted = np.array([9.69233235e-01, 0.24230831j, -4.28344935e-02, -0.00618204j, 7.80904967e-04])
+    assert np.max(np.abs(psi - quoted)) < 1e-8
+    dm = coherent_dm(3, 0.25j)
+    quoted_dm = np.array([[0.93941695, -0.23480733j, -0.04216943],
+                          [0.23480733j, 0.05869011, -0.01054025j],
+                          [-0.04216943, 0.01054025j, 0.00189294]])
+    assert np.max(np.abs(dm - quoted_dm)) < 1e-8
+    assert abs(np.trace(dm) - 1) < 1e-13 and np.allclose(dm, dm.conj().T)
+    alpha, N = 0.7 - 0.4j, 40
+    exact = np.array([np.exp(-abs(alpha) ** 2 / 2) * alpha ** k / np.sqrt(float(factorial(k))) for k in range(N)])
+    assert np.max(np.abs(coherent(N, alpha) - exact)) < 1e-12
+    assert abs(np.linalg.norm(coherent(6, 1.5)) - 1) < 1e-13        # truncated: still normalised

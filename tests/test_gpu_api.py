@@ -68,6 +68,10 @@ def test_evolve_return_result_false_writes_obs_dat(tmp_path, monkeypatch):
     assert len(lines) == 20
     t, v = lines[-1].split()
     assert abs(float(t) - 1.0) < 1e-12 and abs(complex(v) - ref_obs[-1, 0]) < 1e-6
+    from lime_amd import datfile
+    tt, obs = datfile.read_obs("obs.dat")
+    assert np.array_equal(tt, datfile.time_column(0.05, 20))
+    assert obs.shape == (20, 1) and np.max(np.abs(obs[:, 0] - ref_obs[:, 0])) < 1e-6
 
 
 def test_correlation_2op_1t_and_cor_dat(tmp_path, monkeypatch):
@@ -266,15 +270,19 @@ def test_correlation_3p_1t_writes_lime_files(golden_dir, tmp_path, monkeypatch):
     w = np.exp(-np.arange(n) / n_th)
     rho0 = sp.csr_matrix(np.diag(w / w.sum()).astype(np.complex128))
     assert correlation_3p_1t(H, rho0, [a.conj().T, num, a], c_ops, np.linspace(0, 10, 200), lindblad) is None
+    from lime_amd import datfile
     cor_lines = open("cor.dat").read().splitlines()
     dm_lines = open("dm.dat").read().splitlines()
     assert len(cor_lines) == 200 and len(dm_lines) == 200
-    t = np.array([float(l.split()[0]) for l in cor_lines])
-    cor = np.array([complex(l.split()[1]) for l in cor_lines])
-    dm = np.array([[complex(x) for x in l.split()[1:]] for l in dm_lines])
-    assert dm.shape == (200, n * n)
-    assert np.allclose(t, g["t_cor"], rtol=0, atol=1e-12)
+    # the time column is byte-identical to lime's files (t accumulated by t += dt)
+    assert [l.split()[0] for l in cor_lines] == list(g["t_cor_text"])
+    assert [l.split()[0] for l in dm_lines] == list(g["t_dm_text"])
+    assert all(l.endswith(" ") for l in cor_lines + dm_lines)      # lime's trailing blank
+    t, cor = datfile.read_cor("cor.dat")
+    t_dm, dm = datfile.read_dm("dm.dat")
+    assert dm.shape == (200, n, n)
+    assert np.array_equal(t, g["t_cor"]) and np.array_equal(t_dm, g["t_dm"])
     assert np.max(np.abs(cor - g["cor"])) < 2e-6
-    assert np.max(np.abs(dm.reshape(200, n, n) - g["dm"])) < 5e-7
+    assert np.max(np.abs(dm - g["dm"])) < 5e-7
     with pytest.raises(ValueError):
         correlation_3p_1t(H, rho0, [a.conj().T, num, a], c_ops, np.linspace(0, 10, 200), np.sin)

@@ -1,5 +1,11 @@
-"""Ragged batches in one captured graph: every instance against the oracle, graph replays
-against eager launches (several replays, eager work and synchronisations in between)."""
+"""Uneven batches on the GPU.
+
+``RaggedLindblad`` advances every instance in ONE persistent launch: every size class of
+the uneven-batch workload (n = 16, 32, ..., 256, four dense jump operators plus on-site
+dephasing) against the FP64 oracle under every launch shape; graph replays against eager
+launches bit for bit (several replays, eager work and synchronisations in between); the
+result independent of how many workgroups drain the queue.  ``RaggedClasses`` (size
+classes captured into one graph) is checked the same way."""
 import numpy as np
 import pytest
 
@@ -9,9 +15,9 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-def small_ragged(seed=31):
+def ragged(seed, sizes, count=None):
     from lime_amd import configs
-    return configs.ragged_gue(seed=seed, count=14, sizes=(8, 24, 40, 64, 100, 130), fro=0.1)
+    return configs.ragged_gue(seed=seed, count=count or len(sizes), sizes=tuple(sizes), fro=0.1)
 
 
 def oracle_state(inst, dt, nt):
@@ -21,12 +27,82 @@ def oracle_state(inst, dt, nt):
     return rho
 
 
-def test_ragged_graph_matches_oracle_and_eager():
+def _one_per_class(seed):
+    """One instance of every size class of the workload (n = 16 .. 256 step 16)."""
+    return [ragged(seed + n, (n,))["instances"][0] for n in range(16, 257, 16)]
+
+
+@pytest.fixture(scope="module")
+def class_refs():
+    inst = _one_per_class(41)
+    dt, nt = 0.02, 4
+    return inst, dt, nt, [oracle_state(x, dt, nt) for x in inst]
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_persistent_every_size_class_matches_oracle(class_refs, variant):
     from lime_amd.ragged import RaggedLindblad
-    cfg = small_ragged()
+    inst, dt, nt, refs = class_refs
+    rb = RaggedLindblad(inst, tune=False)
+    assert rb.variants() == 5
+    rb.variant = variant
+    rb.step(dt, nt)
+    got = rb.states()
+    for x, g, ref in zip(inst, got, refs):
+        n = x["H"].shape[0]
+        err = np.max(np.abs(g - ref))
+        print(f"persistent v{variant} n={n}: max error {err:.3e}")
+        assert err < 2e-7, n
+        assert abs(np.trace(g) - 1) < 1e-6
+
+
+def test_persistent_graph_replays_equal_eager_and_any_grid():
+    from lime_amd.ragged import RaggedLindblad
+    cfg = ragged(42, (8, 24, 40, 64, 100, 130), count=14)
+    dt, chunk, reps = 0.02, 3, 3
+    eager = RaggedLindblad(cfg["instances"], tune=False)
+    graph = RaggedLindblad(cfg["instances"], tune=False)
+    one = RaggedLindblad(cfg["instances"], tune=False, max_groups=1)   # one workgroup drains all
+    graph.capture(dt, chunk)
+    for r in range(reps):
+        eager.step(dt, chunk)
+        torch.cuda.synchronize()
+        graph.graph.replay()
+        torch.cuda.synchronize()
+        one.step(dt, chunk)
+        junk = torch.randn(1000, device="cuda").sum()          # unrelated eager work
+        torch.cuda.synchronize()
+        assert torch.equal(eager.rho, graph.rho), f"replay {r}"
+        assert torch.equal(eager.rho, one.rho), f"launch {r}"
+        assert int(graph.queue.abs().sum()) == 0                 # the kernel left the queue empty
+    got = graph.states()
+    for i, x in enumerate(cfg["instances"]):
+        err = np.max(np.abs(got[i] - oracle_state(x, dt, chunk * reps)))
+        print(f"persistent graph instance {i} n={x['H'].shape[0]}: err {err:.3e}")
+        assert err < 2e-7
+
+
+def test_persistent_run_chunks_and_remainder():
+    from lime_amd.ragged import RaggedLindblad
+    cfg = ragged(43, (16, 48, 96, 144))
+    dt = 0.02
+    a = RaggedLindblad(cfg["instances"], tune=False)
+    a.run(dt, 11, chunk=4)                 # 2 replays + 3 eager steps
+    b = RaggedLindblad(cfg["instances"], tune=False)
+    b.step(dt, 11)
+    assert torch.equal(a.rho, b.rho)
+    with pytest.raises(ValueError):
+        bad = dict(cfg["instances"][0])
+        bad["rho0"] = np.eye(3)
+        RaggedLindblad([bad], tune=False)
+
+
+def test_class_graph_matches_oracle_and_eager():
+    from lime_amd.ragged import RaggedClasses
+    cfg = ragged(31, (8, 24, 40, 64, 100, 130), count=14)
     dt, chunk, reps = 0.02, 5, 3
-    eager = RaggedLindblad(cfg["instances"], tune=True, verbose=True)
-    graph = RaggedLindblad(cfg["instances"], tune=True)
+    eager = RaggedClasses(cfg["instances"], tune=True, verbose=True)
+    graph = RaggedClasses(cfg["instances"], tune=True)
     graph.capture(dt, chunk)
     for r in range(reps):
         eager.step_eager(dt, chunk)
@@ -40,18 +116,6 @@ def test_ragged_graph_matches_oracle_and_eager():
     for i, inst in enumerate(cfg["instances"]):
         ref = oracle_state(inst, dt, chunk * reps)
         err = np.max(np.abs(got[i] - ref))
-        print(f"ragged instance {i} n={inst['H'].shape[0]} err {err:.3e}")
+        print(f"ragged classes instance {i} n={inst['H'].shape[0]} err {err:.3e}")
         assert err < 2e-7
     assert graph.flops_per_step > 0
-
-
-def test_ragged_run_chunks_and_remainder():
-    from lime_amd.ragged import RaggedLindblad
-    cfg = small_ragged(seed=32)
-    dt = 0.02
-    a = RaggedLindblad(cfg["instances"], tune=False)
-    a.run(dt, 11, chunk=4)                 # 2 replays + 3 eager steps
-    b = RaggedLindblad(cfg["instances"], tune=False)
-    b.step_eager(dt, 11)
-    for ca, cb in zip(a.classes, b.classes):
-        assert torch.allclose(ca["rho"], cb["rho"], atol=1e-13, rtol=0)
