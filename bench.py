@@ -140,6 +140,8 @@ def main():
                     help="ragged: one persistent launch (default) or the per-class graph")
     ap.add_argument("--check-steps", type=int, default=100,
                     help="dense512: RK4 steps of the FP64 GPU reference comparison")
+    ap.add_argument("--gradient-steps", type=int, default=0,
+                    help="gradient: RK4 steps of the trajectory (default: the workload's 2000)")
     ap.add_argument("--variant", type=int, default=None,
                     help="launch-shape variant (default: autotuned on the workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -647,6 +649,8 @@ def bench_gradient(args):
     tgt = configs.chain_gradient(seed=cfg["seed"] + 1000)
     dev = torch.device("cuda")
     B, ns, nt, K, dt = cfg["eps"].shape[0], cfg["eps"].shape[1], cfg["nt"], cfg["checkpoint"], cfg["dt"]
+    if args.gradient_steps:
+        nt = args.gradient_steps                 # a shorter trajectory (profiling runs)
     mu = torch.tensor(cfg["mu"][:, 0], device=dev)
     prec = args.precision
     with torch.no_grad():

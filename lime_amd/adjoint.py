@@ -35,7 +35,10 @@ from .propagator import LindbladPropagator, device
 
 
 def chain_generator(eps, J, gamma):
-    """H_eff, D (complex128, (B, n, n)) of the chain + ground model on the GPU."""
+    """H_eff, D (complex128, (B, n, n)) of the chain + ground model on the GPU.
+
+    gamma: a scalar (one rate shared by every site and chain), (B,) (one rate per chain) or
+    (1, ns) (one rate per site, shared by the chains)."""
     B, ns = eps.shape
     n = ns + 1
     dev = eps.device
@@ -44,13 +47,16 @@ def chain_generator(eps, J, gamma):
     H[:, idx, idx] = eps.to(torch.complex128)
     H[:, idx[:-1], idx[1:]] = J.to(torch.complex128)
     H[:, idx[1:], idx[:-1]] = J.to(torch.complex128)
-    site = torch.zeros(n, dtype=torch.float64, device=dev)
-    site[1:] = 1.0
-    g = gamma.to(torch.float64).reshape(-1, 1) if torch.is_tensor(gamma) else torch.full((B, 1), float(gamma), dtype=torch.float64, device=dev)
-    if g.shape[0] == 1:
-        g = g.expand(B, 1)
-    heff = H - 0.5j * torch.diag_embed((g * site).to(torch.complex128))
-    dmat = torch.diag_embed((g * site).to(torch.complex128))
+    g = torch.as_tensor(gamma, dtype=torch.float64, device=dev)
+    rates = torch.zeros((B, n), dtype=torch.float64, device=dev)
+    if g.dim() == 2:
+        if g.shape != (1, ns):
+            raise ValueError(f"per-site dephasing rates must have shape (1, {ns}), got {tuple(g.shape)}")
+        rates[:, 1:] = g
+    else:
+        rates[:, 1:] = g.reshape(-1, 1) if g.dim() == 1 else g
+    heff = H - 0.5j * torch.diag_embed(rates.to(torch.complex128))
+    dmat = torch.diag_embed(rates.to(torch.complex128))
     return heff.contiguous(), dmat.contiguous()
 
 
@@ -76,7 +82,7 @@ class ChainQRT:
         self.B, self.ns = self.eps.shape
         self.n = self.ns + 1
         self.dt, self.nt, self.K = float(dt), int(nt), int(checkpoint)
-        heff, dmat = chain_generator(self.eps, self.J, torch.as_tensor(gamma, device=self.dev))
+        heff, dmat = chain_generator(self.eps, self.J, gamma)
         self.prop = LindbladPropagator.from_folded(heff, dmat, force_tiled=force_tiled,
                                                    precision=precision)
         self.x0 = torch.zeros((self.B, self.n, self.n), dtype=torch.complex128, device=self.dev)
@@ -132,7 +138,8 @@ class ChainQRT:
         return out, stages
 
     def backward(self, G):
-        """G (B, Nt) complex: torch's dL/dC^*.  Returns (g_eps (B,ns), g_J (B,ns-1), g_gamma (B,))."""
+        """G (B, Nt) complex: torch's dL/dC^*.  Returns (g_eps (B,ns), g_J (B,ns-1),
+        g_gamma_sites (B,ns): the gradient w.r.t. each site's dephasing rate in each chain)."""
         B, n, h = self.B, self.n, self.dt
         dev = self.dev
         mu_h = self.mu.conj().transpose(0, 1).contiguous()
@@ -181,18 +188,19 @@ class ChainQRT:
                 lam, acc = acc, lam                      # lambda_k = acc; the old buffer is reused
         g_eps = outa[:, 1:].imag.clone()
         g_J = outu[:, 1:].imag.clone()
-        g_gamma = outb[:, 1:].real.sum(dim=1)
-        return g_eps, g_J, g_gamma
+        g_gamma_sites = outb[:, 1:].real.clone()          # d/d(dephasing rate of site j), per chain
+        return g_eps, g_J, g_gamma_sites
 
 
 class ChainAbsorption(torch.autograd.Function):
     """C(t) of the chain + ground model as a differentiable function of (eps, J, gamma).
 
-    forward(eps (B, ns), J (B, ns-1), gamma (scalar or (B,)), mu_sites (ns,), dt, nt, K,
-            precision) -> C (B, nt) complex128.  gamma's gradient is summed over the batch
-    when gamma is a scalar (a shared rate).  precision 'fp32' runs the FP32-MFMA kernels,
-    'fp64' the FP64-MFMA tiled kernels for both passes, 'fp32/fp64' the FP32-MFMA forward
-    and recomputation with FP64-MFMA adjoint applications.
+    forward(eps (B, ns), J (B, ns-1), gamma, mu_sites (ns,), dt, nt, K, precision) -> C (B, nt)
+    complex128.  gamma is a scalar (one shared rate: its gradient sums every site of every
+    chain), (B,) (a rate per chain) or (1, ns) (a rate per site, shared by the chains).
+    precision 'fp32' runs the FP32-MFMA kernels, 'fp64' the FP64-MFMA tiled kernels for both
+    passes, 'fp32/fp64' the FP32-MFMA forward and recomputation with FP64-MFMA adjoint
+    applications.
     """
 
     @staticmethod
@@ -207,9 +215,15 @@ class ChainAbsorption(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_C):
-        g_eps, g_J, g_gamma = ctx.q.backward(grad_C)
-        gg = g_gamma.sum() if len(ctx.gamma_shape) == 0 else g_gamma
-        return g_eps, g_J, gg.reshape(ctx.gamma_shape), None, None, None, None, None
+        g_eps, g_J, g_sites = ctx.q.backward(grad_C)
+        shape = ctx.gamma_shape
+        if len(shape) == 0:
+            gg = g_sites.sum()
+        elif len(shape) == 1:
+            gg = g_sites.sum(dim=1)
+        else:
+            gg = g_sites.sum(dim=0, keepdim=True)
+        return g_eps, g_J, gg.reshape(shape), None, None, None, None, None
 
 
 def spectrum_loss(C, target, dt):
