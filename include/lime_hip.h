@@ -58,7 +58,7 @@ extern "C" {
  *   L_k = the dense (non-diagonal) jump operators; R_k = L_k unless `rops` is given
  *   (general sandwich terms, e.g. the Hilbert-space form of the Redfield tensor).
  *
- * FP32 with n <= 96 (n <= 64 with dense operators) runs the resident kernel: one
+ * FP32 with n <= 80 (n <= 64 with dense operators) runs the resident kernel: one
  * workgroup per operator for all steps, no workspace; n = 16m + 1 without dense operators
  * uses its bordered layout.  Larger n, PREC_FP64, PREC_3XBF16 and FORCE_TILED run one
  * launch per RK4 stage and need `work` of lime_lindblad_workspace_bytes() bytes
@@ -86,7 +86,7 @@ typedef struct lime_lindblad_batch {
   void* stage_states;  /* complex128 [rec_capacity][3][batch][n][n] (record-major) or NULL:
                           the FP64 RK4 stage operands X + h/2 k1, X + h/2 k2, X + h k3 of
                           every recorded step (per-stage kernels only: set
-                          LIME_F_FORCE_TILED for n <= 96)                                  */
+                          LIME_F_FORCE_TILED for n <= 80)                                  */
 } lime_lindblad_batch;
 
 typedef struct lime_step_params {

@@ -8,7 +8,7 @@ Model (Hilbert space = ground state |g> = index 0 + n_sites sites):
     H = sum_j eps_j |j><j| + J_j (|j><j+1| + h.c.),  L_j = sqrt(gamma) |j><j|  (sites only),
     mu = sum_j mu_j (|j><g| + |g><j|),  X_0 = mu |g><g|,  C_k = Tr[mu X_k],  k = 1..Nt.
 
-Forward: the fused kernel (resident for n <= 96, tiled above; launch shape from the
+Forward: the fused kernel (resident for n <= 80, tiled above; launch shape from the
 autotuner) records C_k every step, K steps per launch; the state after each launch is the
 checkpoint X_{cK} (default K = 64).
 

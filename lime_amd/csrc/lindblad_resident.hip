@@ -162,7 +162,7 @@ lindblad_rk4_resident(ResidentArgs a) {
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int n = a.n;
   const int set = b / a.group;
-  const int nn = n * n;   // n <= 96: element offsets fit in 32 bits
+  const int nn = n * n;   // n <= 80: element offsets fit in 32 bits
   const int lr = lane & 15;
   const int lg = lane >> 4;
   // reduction extent: operands are zero beyond n, so round n up to the MFMA k-step
@@ -797,7 +797,7 @@ constexpr Shape kShapes[7][kMaxVariants] = {
     {{9, 1}, {5, 2}, {0, 0}, {0, 0}},
     {{16, 1}, {8, 2}, {0, 0}, {0, 0}},
     {{13, 2}, {0, 0}, {0, 0}, {0, 0}},
-    {{12, 3}, {0, 0}, {0, 0}, {0, 0}},
+    {},
 };
 constexpr Shape kBorderShapes[7][kMaxVariants] = {
     {},
@@ -806,7 +806,7 @@ constexpr Shape kBorderShapes[7][kMaxVariants] = {
     {{4, 1}, {8, 1}, {0, 0}, {0, 0}},
     {{9, 1}, {0, 0}, {0, 0}, {0, 0}},
     {{16, 1}, {0, 0}, {0, 0}, {0, 0}},
-    {{13, 2}, {0, 0}, {0, 0}, {0, 0}},
+    {},
 };
 
 template <int NB, int W, int BPW, int MINW, bool DENSE, bool REC, bool BRD, bool DB>
@@ -837,10 +837,11 @@ hipError_t launch_rec(const ResidentArgs& a, size_t lds, hipStream_t stream, boo
 template <int NB, int W, int BPW>
 hipError_t launch_shape(const ResidentArgs& a, size_t lds, hipStream_t stream, bool db) {
   static_assert(W * 64 <= 1024, "a workgroup holds at most 16 waves");
-  constexpr int MINW = BPW == 1 ? 6 : BPW == 2 ? 4 : BPW == 3 ? 3 : 2;
-  // the dense products stage T = L S and their own accumulators: one wave per SIMD fewer
-  constexpr int MINW_D = (BPW == 1 ? MINW - 2 : MINW - 1) > (W + 3) / 4 ? (BPW == 1 ? MINW - 2 : MINW - 1)
-                                                                         : (W + 3) / 4;
+  // two tiles per wave: 4 waves per SIMD only where the workgroup needs them (W > 12)
+  constexpr int MINW = BPW == 1 ? 6 : BPW == 2 ? (W > 12 ? 4 : 3) : BPW == 3 ? 3 : 2;
+  // the dense products stage T = L S and their own accumulators: two waves per SIMD fewer
+  // for one tile per wave
+  constexpr int MINW_D = (BPW == 1 ? MINW - 2 : MINW) > (W + 3) / 4 ? (BPW == 1 ? MINW - 2 : MINW) : (W + 3) / 4;
   if constexpr (NB <= 4) {
     if (a.n_dense > 0) return launch_rec<NB, W, BPW, MINW_D, true, false>(a, lds, stream, false);
   } else {
@@ -879,9 +880,12 @@ size_t lds_layout(int n, int variant, int n_dense, int n_eops, int nbuf) {
 
 }  // namespace
 
-int resident_nb_max(int n_dense) { return n_dense > 0 ? 4 : 6; }
+// n <= 80 (64 with dense operators).  At 81..96 (NB = 6) the FP64 state of 3 tiles per wave
+// (or 2 with the border outputs) does not fit the register budget of the 12-13 waves those
+// sizes need: the builds spilled 20-92 VGPRs, so those sizes run the per-stage kernels.
+int resident_nb_max(int n_dense) { return n_dense > 0 ? 4 : 5; }
 
-bool resident_bordered(int n, int n_dense) { return n_dense == 0 && n >= 17 && n % 16 == 1 && n <= 81; }
+bool resident_bordered(int n, int n_dense) { return n_dense == 0 && n >= 17 && n % 16 == 1 && n <= 65; }
 
 int resident_variants(int n, int n_dense) {
   const int nb = cdiv(n, 16);
@@ -916,7 +920,6 @@ hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t strea
       case 31: return launch_border<3, 8, 1>(a, lds, stream, db);
       case 40: return launch_border<4, 9, 1>(a, lds, stream, db);
       case 50: return launch_border<5, 16, 1>(a, lds, stream, db);
-      case 60: return launch_border<6, 13, 2>(a, lds, stream, db);
       default: return hipErrorInvalidValue;
     }
   }
@@ -929,7 +932,6 @@ hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t strea
     case 40: return launch_shape<4, 16, 1>(a, lds, stream, db);
     case 41: return launch_shape<4, 8, 2>(a, lds, stream, db);
     case 50: return launch_shape<5, 13, 2>(a, lds, stream, db);
-    case 60: return launch_shape<6, 12, 3>(a, lds, stream, db);
     default: return hipErrorInvalidValue;
   }
 }

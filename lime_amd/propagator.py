@@ -113,7 +113,7 @@ class LindbladPropagator:
         """``sandwich``: optional explicit (L, R) lists of (S,N,N) operators contributing
         sum_k L_k X R_k^dag (used instead of folding dense jump operators; H must then
         already be the effective generator).  ``force_tiled`` selects the per-stage tiled
-        kernels even for n <= 96."""
+        kernels even for n <= 80."""
         self.dev = device()
         self.force_tiled = bool(force_tiled)
         self.one_sided = False
@@ -296,7 +296,7 @@ class LindbladPropagator:
 
         ``stage_states`` (rec_capacity, 3, B, N, N) complex128 (record-major) receives the
         FP64 stage operands X + h/2 k1, X + h/2 k2, X + h k3 of every recorded step
-        (per-stage kernels: the propagator must be ``force_tiled`` for n <= 96)."""
+        (per-stage kernels: the propagator must be ``force_tiled`` for n <= 80)."""
         n, B = self.n, self.batch
         _check(rho, "rho", (B, n, n), torch.complex128, self.dev)
         eop_group = int(eop_group or B)

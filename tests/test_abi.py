@@ -36,7 +36,7 @@ def test_every_declared_symbol_is_exported(lib):
 
 def test_version_and_limits(lib):
     assert b"gfx950" in lib.lime_version()
-    assert lib.lime_lindblad_max_n(0, _lib.LIME_F_PREC_FP32) == 96
+    assert lib.lime_lindblad_max_n(0, _lib.LIME_F_PREC_FP32) == 80
     assert lib.lime_lindblad_max_n(3, _lib.LIME_F_PREC_FP32) == 64
     d = _lib.LindbladBatch()
     d.batch, d.n, d.group, d.eop_group, d.flags = 2, 20, 1, 1, _lib.LIME_F_PREC_FP64
@@ -73,7 +73,7 @@ def test_argument_validation_without_gpu(lib):
     # 3xBF16 runs on the tiled kernels at every n: it needs a workspace too
     assert _call(lib, flags=_lib.LIME_F_PREC_3XBF16) == _lib.LIME_ERR_ARG
     assert b"workspace" in lib.lime_last_error()
-    # n > 96 runs the tiled kernels, which need a caller-provided workspace
+    # n > 80 runs the tiled kernels, which need a caller-provided workspace
     assert _call(lib, n=97) == _lib.LIME_ERR_ARG
     assert b"workspace" in lib.lime_last_error()
     # trace renormalisation on the tiled kernels needs 8 bytes per operator more workspace

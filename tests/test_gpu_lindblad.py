@@ -171,7 +171,7 @@ def test_chunked_run_equals_single_launch():
 @pytest.mark.parametrize("n,kd,tiled", [(10, 0, True), (33, 2, True), (64, 0, True),
                                          (65, 1, True), (130, 0, False), (150, 2, False)])
 def test_tiled_path_matches_oracle(n, kd, tiled):
-    """Per-stage tiled kernels (forced at small n, automatic above 96), incl. dense L,
+    """Per-stage tiled kernels (forced at small n, automatic above 80), incl. dense L,
     observables (FP64 atomics) and state records."""
     rng = np.random.default_rng(300 + n)
     H = rand_herm(rng, n)
