@@ -54,6 +54,8 @@ def profiled_traffic(args, variant):
     kernel source differ from the profiled ones (the figure would be stale)."""
     if (args.realisations, args.rk_per_step) != (4096, 256):
         return None, "not the profiled configuration (4096 realisations, 256 steps per launch)"
+    if args.precision != "fp32":
+        return None, f"profiled for the FP32 kernel; this run is {args.precision}"
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "lindblad_rk4_resident_lattice65_traffic.txt")
     fields = {}
@@ -184,8 +186,12 @@ def main():
     R, n = args.realisations, cfg["n"]
     B = 3 * R
     dt, chunk = cfg["dt"], args.rk_per_step
-    log(f"[bench] building operators: {R} realisations x 3 components, n={n}")
-    prop = LindbladPropagator(cfg["H"], cfg["c_ops"], group=3)
+    prec = args.precision
+    if prec not in ("fp32", "3xbf16"):
+        print(f"bench.py: the lattice65 workload runs in fp32 or 3xbf16 (asked for {prec})", file=sys.stderr)
+        sys.exit(2)
+    log(f"[bench] building operators: {R} realisations x 3 components, n={n}, {prec}")
+    prop = LindbladPropagator(cfg["H"], cfg["c_ops"], group=3, precision=prec)
     X = torch.as_tensor(cfg["x0"].reshape(B, n, n), device=dev).contiguous()
     E_t = torch.as_tensor(cfg["mu"].reshape(B, 1, n, n), device=dev).transpose(-1, -2).contiguous()
     total_steps = (args.warmup + args.steps) * chunk
@@ -236,17 +242,37 @@ def main():
 
     extra = {}
     if not args.no_check:
-        extra.update(accuracy_checks(cfg, prop, obs, total_steps, dt, R))
+        extra.update(accuracy_checks(cfg, prop, obs, total_steps, dt, R, prec))
     if not args.no_dense:
-        d = dense_operand_leg(cfg, prop.variant, 3)
+        d = dense_operand_leg(cfg, prop.variant, 3, prec)
         extra["dense_operand"] = d
         extra["dense_operand_rho_steps_per_s"] = d["value"]
     if not args.no_plain:
         extra.update(plain_fp64(cfg, B, dt, value))
 
     traffic, traffic_note = profiled_traffic(args, prop.variant)
+    if prec == "3xbf16":
+        # BF16 MFMA work issued: 4 real products per complex product, 3 split products each,
+        # over the nc = n - 1 core, two complex products per stage, plus the border row and
+        # column (one 16-wide product set per tile row / column: 2 nc / 16 sets of 4 x 3
+        # products of 16 x nc x 16 per complex product pair)
+        nc = n - 1
+        issued_per_rho_step = 4 * (2 * 4 * 3 * 2 * nc ** 3 + (2 * nc // 16) * 4 * 3 * 2 * 16 * nc * 16)
+        issued = {"issued_bf16_tflops": achieved * issued_per_rho_step / fl,
+                  "issued_bf16_frac": achieved * issued_per_rho_step / fl / PEAK_BF16_MFMA_TFLOPS}
+        kernel = "lindblad_rk4_resident_bf3 (bordered n=65, 3xBF16 split products)"
+    else:
+        # FP32 MFMA work the kernel issues: the two complex products in the
+        # three-multiplication form over the nc = n - 1 core, 6 real products
+        # of nc^3 per stage = 48 nc^3 per rho-step (the border row and column
+        # run on VALU), against the canonical 64 n^3
+        issued = {"issued_mfma_tflops": achieved * 48.0 * (n - 1) ** 3 / (64.0 * n ** 3),
+                  "issued_mfma_frac": achieved * 48.0 * (n - 1) ** 3 / (64.0 * n ** 3)
+                  / PEAK_FP32_MFMA_TFLOPS}
+        kernel = f"lindblad_rk4_resident (bordered n=65, variant {prop.variant})"
     result = {
-        "metric": "RK4 rho-steps/s (batched FP32-MFMA Lindblad QRT propagation)",
+        "metric": ("RK4 rho-steps/s (batched FP32-MFMA Lindblad QRT propagation)" if prec == "fp32" else
+                   "RK4 rho-steps/s (batched 3xBF16-MFMA Lindblad QRT propagation)"),
         "value": value,
         "unit": "rho-steps/s",
         "n_gpus": 1,
@@ -256,7 +282,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64-fp32mfma+f64",
+        "dtype": _dtype(prec),
         "data": "synthetic (seeded 8x8 exciton-lattice disorder ensemble)",
         "config": {"workload": "lattice_spectrum: 4096 realisations x 3 dipole components, "
                                "n=65 (64 sites + ground), dt=0.05, C(t) every step",
@@ -268,15 +294,9 @@ def main():
                      "traffic": traffic,
                      "traffic_source": traffic_note + " (rocprofv3 FETCH_SIZE + WRITE_SIZE of "
                                        "the same kernel source and variant, copy-calibrated)",
-                     "kernel": f"lindblad_rk4_resident (bordered n=65, variant {prop.variant})",
+                     "kernel": kernel,
                      "launch_ms_mean": float(np.mean(launch_ms)),
-                     # FP32 MFMA work the kernel issues: the two complex products in the
-                     # three-multiplication form over the nc = n - 1 core, 6 real products
-                     # of nc^3 per stage = 48 nc^3 per rho-step (the border row and column
-                     # run on VALU), against the canonical 64 n^3
-                     "issued_mfma_tflops": achieved * 48.0 * (n - 1) ** 3 / (64.0 * n ** 3),
-                     "issued_mfma_frac": achieved * 48.0 * (n - 1) ** 3 / (64.0 * n ** 3)
-                     / PEAK_FP32_MFMA_TFLOPS},
+                     **issued},
         "spectrum_peak": peak,
     }
     result.update(extra)
@@ -289,7 +309,7 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def accuracy_checks(cfg, prop, obs, total_steps, dt, R):
+def accuracy_checks(cfg, prop, obs, total_steps, dt, R, prec="fp32"):
     """Max errors vs the FP64 oracle on the records of the timed run itself."""
     import torch
     from oracle import lindblad as ol
@@ -331,7 +351,7 @@ def accuracy_checks(cfg, prop, obs, total_steps, dt, R):
     v = cfg["mu"][0, 0][:, 0].copy()
     rho0 = np.outer(v, v.conj()) / np.vdot(v, v)
     P = [np.diag(np.eye(n)[i]).astype(np.complex128) for i in range(n)]
-    p1 = LindbladPropagator(cfg["H"][0], cfg["c_ops"])
+    p1 = LindbladPropagator(cfg["H"][0], cfg["c_ops"], precision=prec)
     rho, pobs, _ = p1.run(rho0, dt, nt, e_ops=P, nout=16)
     _, ref, _ = ol.propagate(cfg["H"][0], rho0, cfg["c_ops"], dt, nt, P, nout=16)
     pe = float(np.max(np.abs(pobs[0].cpu().numpy() - ref)))
@@ -344,7 +364,7 @@ def accuracy_checks(cfg, prop, obs, total_steps, dt, R):
     return out
 
 
-def dense_operand_leg(cfg, variant, group, launches=2, chunk=256, seed=7):
+def dense_operand_leg(cfg, variant, group, prec="fp32", launches=2, chunk=256, seed=7):
     """The headline kernel (same n, batch, launch shape and records) on dense data: GUE
     H_eff sets of the lattice's spectral radius with on-site dephasing, and dense random
     complex operators, so no operand is mostly zeros."""
@@ -360,7 +380,7 @@ def dense_operand_leg(cfg, variant, group, launches=2, chunk=256, seed=7):
     Hs = torch.as_tensor(configs.gue(rng, n, radius)).unsqueeze(0).repeat(R, 1, 1)
     # per-realisation diagonal shift keeps the sets distinct without R eigendecompositions
     Hs = Hs + torch.diag_embed(torch.as_tensor(rng.normal(0, 0.05, (R, n)).astype(np.complex128)))
-    prop = LindbladPropagator(Hs, cfg["c_ops"], group=group)
+    prop = LindbladPropagator(Hs, cfg["c_ops"], group=group, precision=prec)
     prop.variant = variant
     dev = prop.dev
     X0 = (rng.standard_normal((B, n, n)) + 1j * rng.standard_normal((B, n, n))) / n

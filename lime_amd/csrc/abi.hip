@@ -41,9 +41,17 @@ bool is_split_bf16(const lime_lindblad_batch* d) {
   return (d->flags & LIME_F_PREC_MASK) == LIME_F_PREC_3XBF16;
 }
 
+// 3xBF16 on the resident kernel: the bordered sizes n = 33, 49, 65 without dense operators
+bool use_resident_bf3(const lime_lindblad_batch* d) {
+  if (d->flags & (LIME_F_FORCE_TILED | LIME_F_ONE_SIDED)) return false;
+  if (!is_split_bf16(d) || d->rops != nullptr || !lime::resident_bf3_eligible(d->n, d->n_dense)) return false;
+  return lime::resident_bf3_lds_bytes(d->n, d->obs ? d->n_eops : 0) <= kLdsLimit;
+}
+
 bool use_resident(const lime_lindblad_batch* d) {
   if (d->flags & (LIME_F_FORCE_TILED | LIME_F_ONE_SIDED)) return false;
-  // the FP64 and 3xBF16 variants are built for the tiled kernels
+  if (use_resident_bf3(d)) return true;
+  // FP64 (and 3xBF16 at other sizes) run on the tiled kernels
   if ((d->flags & LIME_F_PREC_MASK) != LIME_F_PREC_FP32) return false;
   const int nb = lime::cdiv(d->n, 16);
   if (nb > lime::resident_nb_max(d->n_dense)) return false;
@@ -134,7 +142,7 @@ int rk4_impl(const lime_lindblad_batch* d, const lime_step_params* p, void* stre
     a.nout = (d->obs || d->states) ? p->nout : 0;
     a.step_offset = p->step_offset;
     a.rec_capacity = p->rec_capacity;
-    hipError_t e = lime::resident_launch(a, resident_variant(d), s);
+    hipError_t e = use_resident_bf3(d) ? lime::resident_bf3_launch(a, s) : lime::resident_launch(a, resident_variant(d), s);
     if (e != hipSuccess) return hip_fail(e, (std::string(name) + " (resident) launch").c_str());
     return LIME_OK;
   }
@@ -198,6 +206,7 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1) return 0;
   lime_lindblad_batch probe = *d;
   probe.variant = 0;
+  if (use_resident_bf3(&probe)) return 1;
   if (use_resident(&probe)) return lime::resident_variants(d->n, d->n_dense);
   return 3;   // tiled: 0 = heuristic, 1 = 32x32 tiles, 2 = 64x64 tiles
 }

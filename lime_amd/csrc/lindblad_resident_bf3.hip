@@ -1,0 +1,701 @@
+// Fused batched Lindblad RK4 on 3xBF16 split products, one workgroup per propagated
+// operator, state resident on chip: the split-product arithmetic of lindblad_rk4_resident
+// for the bordered size n = 65 (an 8 x 8 lattice or a 64-site chain plus the ground state).
+//
+// Same right-hand side and time stepping as lindblad_rk4_resident (lime/oqs.py:706-723,
+// lime/phys.py:636-649): f(X) = -i (H_eff X - X H_eff^dag) + D o X, RK4 with the FP64 state
+// in VGPRs.  The two complex products run on v_mfma_f32_16x16x32_bf16: every real operand is
+// split x = hi + lo (two BF16 values, hi rounded to nearest even, lo = x - hi rounded again,
+// so |x - hi - lo| <= 2^-17 |x|) and every real product X Y is accumulated in FP32 as
+// Xhi Yhi + Xhi Ylo + Xlo Yhi.
+//
+// Work decomposition (n = NC + 1, NC = 64; 8 waves, two 16 x 16 core tiles each):
+//   * LDS holds the split operands as BF16 planes, one per part (Re hi, Re lo, Im hi, Im lo):
+//     H_eff by rows (A operand of H S; B operand of S H^dag, whose B[k][j] = conj(H[j][k]));
+//     S by rows (A of S H^dag) and by columns (B of H S).  An MFMA operand fragment is 8
+//     consecutive k of one plane row: one 16-byte LDS read.  The lanes that own S elements
+//     write both layouts in the epilogue of every stage.
+//   * C = H S - S H^dag over k < NC on MFMA, 4 real products per complex product into split-
+//     sign FP32 accumulators (no negated operand in the MFMA stream); the border index
+//     k = NC is an FP32 VALU term from side arrays (row and column NC of H and S).
+//   * Border outputs ride on the core fragments.  Row NC, columns of tile t: a 16-row A
+//     operand whose rows 0 and 1 hold the (Re, Im) coefficients built from H[NC][:] and
+//     S[NC][:] meets the B fragments of tile column t (4 products) and leaves Re C[NC][j]
+//     in row 0, Im C[NC][j] in row 1 of one accumulator.  Column NC, rows of tile t: a
+//     16-column B operand built from S[:][NC] and H[NC][:] meets the A fragments of tile
+//     row t the same way.  The corner is a VALU wave sum.
+//   * D o S is folded into the accumulators' start values by the epilogue (as in the FP32
+//     kernel); the RK4 combination and the state update are FP64, the increment FP32.
+#include "common.hpp"
+#include "lindblad_resident_decl.hpp"
+
+namespace lime {
+
+namespace {
+
+typedef unsigned short u16;
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ floatx4 mfma_bf(u16x8 a, u16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
+}
+
+// c += X Y for split operands X = xh + xl, Y = yh + yl (the lo*lo term, <= 2^-16 |X Y|
+// elementwise, is dropped); small terms first
+__device__ __forceinline__ floatx4 mfma3(u16x8 xh, u16x8 xl, u16x8 yh, u16x8 yl, floatx4 c) {
+  c = mfma_bf(xl, yh, c);
+  c = mfma_bf(xh, yl, c);
+  return mfma_bf(xh, yh, c);
+}
+
+// Split a pair of floats: h = hi(a) | hi(b) << 16, l = lo(a) | lo(b) << 16 with x = hi + lo
+// (v_cvt_pk_bf16_f32 rounds to nearest even; x - hi is exact in FP32).
+__device__ __forceinline__ void split2(float a, float b, unsigned& h, unsigned& l) {
+  const float2v x = {a, b};
+  const bf16x2 hb = __builtin_convertvector(x, bf16x2);
+  const float2v hf = __builtin_convertvector(hb, float2v);
+  const bf16x2 lb = __builtin_convertvector(x - hf, bf16x2);
+  h = __builtin_bit_cast(unsigned, hb);
+  l = __builtin_bit_cast(unsigned, lb);
+}
+
+__device__ __forceinline__ u16 neg(unsigned x) { return (u16)(x ^ 0x8000u); }
+
+__device__ __forceinline__ float2 cmul(float2 d, float2 s) {
+  return make_float2(d.x * s.x - d.y * s.y, d.x * s.y + d.y * s.x);
+}
+
+// a b - c conj(d)
+__device__ __forceinline__ float2 border_term(float2 a, float2 b, float2 c, float2 d) {
+  return make_float2(a.x * b.x - a.y * b.y - (c.x * d.x + c.y * d.y),
+                     a.x * b.y + a.y * b.x - (c.y * d.x - c.x * d.y));
+}
+
+// Border coefficient operands (8 of them, each (hi, lo) x 2 rows of PITCH; operand m, part
+// p, row r at row index 4 m + 2 p + r).  With h = g = H[NC][k], s = S[NC][k], c = S[k][NC]:
+//   row outputs  C[NC][j]: A rows (Re coefficient; Im coefficient)
+//     m = 0  [ hx;  hy] with B = Re S       m = 1  [-hy;  hx] with B = Im S
+//     m = 2  [-sx; -sy] with B = Re H'      m = 3  [-sy;  sx] with B = Im H'   (H'[k][j] = H[j][k])
+//   column outputs C[i][NC]: B columns (Re coefficient, Im coefficient)
+//     m = 4  [ cx,  cy] with A = Re H       m = 5  [-cy,  cx] with A = Im H
+//     m = 6  [-gx,  gy] with A = Re S       m = 7  [-gy, -gx] with A = Im S
+template <int PITCH>
+__device__ __forceinline__ void put_coef(u16* sK, int m, int k, u16 r0h, u16 r0l, u16 r1h, u16 r1l) {
+  u16* p = sK + 4 * m * PITCH + k;
+  p[0] = r0h;
+  p[PITCH] = r1h;
+  p[2 * PITCH] = r0l;
+  p[3 * PITCH] = r1l;
+}
+
+// the constant operands (m = 0, 1, 6, 7) from H[NC][k]
+template <int PITCH>
+__device__ __forceinline__ void put_coef_h(u16* sK, int k, float2 v) {
+  unsigned h, l;
+  split2(v.x, v.y, h, l);
+  const unsigned xh = h & 0xffffu, yh = h >> 16, xl = l & 0xffffu, yl = l >> 16;
+  put_coef<PITCH>(sK, 0, k, xh, xl, yh, yl);
+  put_coef<PITCH>(sK, 1, k, neg(yh), neg(yl), xh, xl);
+  put_coef<PITCH>(sK, 6, k, neg(xh), neg(xl), yh, yl);
+  put_coef<PITCH>(sK, 7, k, neg(yh), neg(yl), neg(xh), neg(xl));
+}
+
+// m = 2, 3 from S[NC][k]
+template <int PITCH>
+__device__ __forceinline__ void put_coef_srow(u16* sK, int k, float2 v) {
+  unsigned h, l;
+  split2(v.x, v.y, h, l);
+  const unsigned xh = h & 0xffffu, yh = h >> 16, xl = l & 0xffffu, yl = l >> 16;
+  put_coef<PITCH>(sK, 2, k, neg(xh), neg(xl), neg(yh), neg(yl));
+  put_coef<PITCH>(sK, 3, k, neg(yh), neg(yl), xh, xl);
+}
+
+// m = 4, 5 from S[k][NC]
+template <int PITCH>
+__device__ __forceinline__ void put_coef_scol(u16* sK, int k, float2 v) {
+  unsigned h, l;
+  split2(v.x, v.y, h, l);
+  const unsigned xh = h & 0xffffu, yh = h >> 16, xl = l & 0xffffu, yl = l >> 16;
+  put_coef<PITCH>(sK, 4, k, xh, xl, yh, yl);
+  put_coef<PITCH>(sK, 5, k, neg(yh), neg(yl), xh, xl);
+}
+
+// Wave roles.  n = 65: the 4 x 4 core tiles go two per wave to 8 waves, wave w taking
+// tiles (ti, 2c) and (ti, 2c + 1) with ti = w / 2, c = w % 2, so both share the A
+// fragments of tile row ti.  Each wave also forms one set of border outputs: the row
+// outputs of tile column ti when c = ti / 2 (that column is one of its own, tile u = ti % 2,
+// so the B fragments are shared), else the column outputs of tile row ti (A fragments
+// shared).  The corner goes to lane 2 of wave 1 (a column-output wave, whose lanes
+// lr >= 2 own nothing).
+struct Bf3Layout {
+  static constexpr int TG = 4;                       // core tile grid edge
+  static constexpr int W = 8;                        // waves, two core tiles each
+  static constexpr int NC = 64;                      // core size = border index (n = 65)
+  static constexpr int KP = 64;                      // MFMA reduction extent
+  static constexpr int PITCH = KP + 8;               // BF16 per plane row: 16-byte rows, and
+                                                     // 16 rows at one k hit distinct banks
+  static constexpr int PLANE = NC * PITCH;
+  static constexpr int NCP = NC + 2;                 // side-array length (keeps 16-byte alignment)
+  static constexpr int NBO = 2 * NC + 2;             // border-output slots (2 NC + 1 used)
+  static constexpr size_t kBf16Bytes = (size_t)(12 * PLANE + 32 * PITCH) * 2;
+  // + side arrays (4 x NCP complex64) + border-output state (FP64 rho, FP32 increment, D)
+  static constexpr size_t kFixedBytes = kBf16Bytes + (size_t)4 * NCP * 8 + (size_t)NBO * 32;
+  static constexpr int CORNER_WAVE = 1, CORNER_LANE = 2;
+};
+
+}  // namespace
+
+template <bool REC>
+__global__ void __launch_bounds__(Bf3Layout::W * 64) __attribute__((amdgpu_waves_per_eu(2)))
+lindblad_rk4_resident_bf3(ResidentArgs a) {
+  using L = Bf3Layout;
+  constexpr int W = L::W, NC = L::NC, KP = L::KP, PITCH = L::PITCH, PLANE = L::PLANE;
+  constexpr int T = W * 64;
+
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  u16* sH = reinterpret_cast<u16*>(lds_raw);   // H_eff rows: planes Re hi, Re lo, Im hi, Im lo
+  u16* sR = sH + 4 * PLANE;                    // S rows
+  u16* sC = sR + 4 * PLANE;                    // S columns: plane row j holds S[k][j]
+  u16* sK = sC + 4 * PLANE;                    // border coefficient operands (32 rows)
+  float2* hcol = reinterpret_cast<float2*>(lds_raw + L::kBf16Bytes);   // H[r][NC]
+  float2* hrow = hcol + L::NCP;                                         // H[NC][k]
+  float2* srow = hrow + L::NCP;                                         // S[NC][k]
+  float2* scol = srow + L::NCP;                                         // S[r][NC]
+  // border outputs' state, one slot each (row output j: slot j; column output i: NC + i;
+  // the corner: 2 NC), read and written only by the owning lane
+  double2* bRho = reinterpret_cast<double2*>(scol + L::NCP);
+  float2* bInc = reinterpret_cast<float2*>(bRho + L::NBO);
+  float2* bD = bInc + L::NBO;
+  double2* sRed = reinterpret_cast<double2*>(lds_raw + L::kFixedBytes);
+
+  const int b = xcd_block();   // the operators of one generator set run on one XCD (shared L2)
+  if (b >= a.batch) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int n = a.n;   // NC + 1
+  const int set = b / a.group;
+  const int nn = n * n;
+  const int lr = lane & 15;
+  const int lg = lane >> 4;
+  const int ti = w >> 1, c2 = w & 1;           // tile row, column pair (wave-uniform)
+  const int I0 = 16 * ti, J0 = 32 * c2;        // the wave's tiles: columns J0 .. J0 + 31
+  const bool rowsk = c2 == (ti >> 1);          // row outputs of tile column ti (tile u = ti & 1)
+  const bool colsk = !rowsk;                   // column outputs of tile row ti
+  const int usk = ti & 1;
+
+  // ---- LDS: zero the BF16 region (k >= NC stays zero), then fill it
+  {
+    uint4* z = reinterpret_cast<uint4*>(lds_raw);
+    for (int i = tid; i < (int)(L::kBf16Bytes / 16); i += T) z[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  __syncthreads();
+  {
+    const float2* H = a.heff + (size_t)set * nn;
+    const double2* R = a.rho + (size_t)b * nn;
+    for (int idx = tid; idx < NC * (NC / 2); idx += T) {   // element pairs (i, k), (i, k + 1)
+      const int i = idx / (NC / 2), k = 2 * (idx - i * (NC / 2));
+      unsigned xh, xl, yh, yl;
+      const float2 h0 = H[i * n + k], h1 = H[i * n + k + 1];
+      split2(h0.x, h1.x, xh, xl);
+      split2(h0.y, h1.y, yh, yl);
+      u16* hr = sH + i * PITCH + k;
+      *reinterpret_cast<unsigned*>(hr) = xh;
+      *reinterpret_cast<unsigned*>(hr + PLANE) = xl;
+      *reinterpret_cast<unsigned*>(hr + 2 * PLANE) = yh;
+      *reinterpret_cast<unsigned*>(hr + 3 * PLANE) = yl;
+      const double2 r0 = R[i * n + k], r1 = R[i * n + k + 1];
+      split2((float)r0.x, (float)r1.x, xh, xl);
+      split2((float)r0.y, (float)r1.y, yh, yl);
+      u16* rr = sR + i * PITCH + k;
+      *reinterpret_cast<unsigned*>(rr) = xh;
+      *reinterpret_cast<unsigned*>(rr + PLANE) = xl;
+      *reinterpret_cast<unsigned*>(rr + 2 * PLANE) = yh;
+      *reinterpret_cast<unsigned*>(rr + 3 * PLANE) = yl;
+      u16* c0 = sC + k * PITCH + i;   // S[i][k] -> column plane row k, position i
+      u16* c1 = c0 + PITCH;           // S[i][k + 1]
+      c0[0] = (u16)xh;
+      c1[0] = (u16)(xh >> 16);
+      c0[PLANE] = (u16)xl;
+      c1[PLANE] = (u16)(xl >> 16);
+      c0[2 * PLANE] = (u16)yh;
+      c1[2 * PLANE] = (u16)(yh >> 16);
+      c0[3 * PLANE] = (u16)yl;
+      c1[3 * PLANE] = (u16)(yl >> 16);
+    }
+    for (int k = tid; k <= NC; k += T) {
+      const float2 hc = H[k * n + NC], hr = H[NC * n + k];
+      const double2 rr = R[NC * n + k], rc = R[k * n + NC];
+      const float2 sr = make_float2((float)rr.x, (float)rr.y), sc = make_float2((float)rc.x, (float)rc.y);
+      hcol[k] = hc;
+      hrow[k] = hr;
+      srow[k] = sr;
+      scol[k] = sc;
+      if (k < NC) {
+        put_coef_h<PITCH>(sK, k, hr);
+        put_coef_srow<PITCH>(sK, k, sr);
+        put_coef_scol<PITCH>(sK, k, sc);
+      }
+    }
+  }
+
+  // ---- border outputs owned by this lane (row outputs: lanes 0..15; column outputs: lanes
+  // with lr < 2, rows 4 lg + 2 lr + u of tile row ti; the corner)
+  const bool own_row = rowsk && lane < 16;
+  const bool own_col = colsk && lr < 2;
+  const bool own_corner = w == L::CORNER_WAVE && lane == L::CORNER_LANE;
+  const int nbo = own_col ? 2 : (own_row || own_corner) ? 1 : 0;
+  auto bi = [&](int u) { return own_col ? I0 + 4 * lg + 2 * lr + u : NC; };
+  auto bj = [&](int u) { return own_row ? I0 + lane : NC; };   // row outputs: columns of tile column ti
+  auto bslot = [&](int u) { return own_row ? I0 + lane : own_col ? NC + I0 + 4 * lg + 2 * lr + u : 2 * NC; };
+
+  // FP64 state of the owned core elements (I0 + 4 lg + v, J0 + 16 u + lr), the FP32 RK4
+  // increment and D there
+  double2 rho[2][4];
+  float2 inc[2][4];
+  float2 dc[2][4];
+  const bool has_d = a.dmat != nullptr;
+  {
+    const double2* R = a.rho + (size_t)b * nn;
+    const float2* Dm = a.dmat + (size_t)set * nn;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = J0 + 16 * u + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = I0 + 4 * lg + v;
+        rho[u][v] = R[i * n + j];
+        inc[u][v] = make_float2(0.f, 0.f);
+        dc[u][v] = has_d ? Dm[i * n + j] : make_float2(0.f, 0.f);
+      }
+    }
+    for (int u = 0; u < nbo; ++u) {
+      const int x = bslot(u);
+      bRho[x] = R[bi(u) * n + bj(u)];
+      bInc[x] = make_float2(0.f, 0.f);
+      bD[x] = has_d ? Dm[bi(u) * n + bj(u)] : make_float2(0.f, 0.f);
+    }
+  }
+  __syncthreads();
+
+  // accumulators of C: Re C = rp - rn, Im C = ip - in.  The stage derivative is k = -i C
+  // (+ D o S folded in: C starts at i (D o S), i.e. Re C = -Im(D o S), Im C = Re(D o S))
+  floatx4 rp[2], rn[2], ip[2], in[2], q;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float2 p = cmul(dc[u][v], make_float2((float)rho[u][v].x, (float)rho[u][v].y));
+      rp[u][v] = -p.y;
+      ip[u][v] = p.x;
+    }
+
+  const double dt = a.dt;
+  const bool renorm = (a.flags & 1) != 0;
+  const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
+  const int aoff = (I0 + lr) * PITCH + 8 * lg;   // fragments of tile row ti (A operands)
+  const int boff = (J0 + lr) * PITCH + 8 * lg;   // fragments of the wave's first tile column (B)
+  const int koff = (lr & 1) * PITCH + 8 * lg;    // coefficient rows (rows 2..15 repeat 0, 1:
+                                                 // their outputs are not used)
+  auto frag = [&](const u16* base, int off) { return *reinterpret_cast<const u16x8*>(base + off); };
+
+  for (int s = 0; s < a.nsteps; ++s) {
+#pragma unroll 1
+    for (int st = 0; st < 4; ++st) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        rn[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        in[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+      q = floatx4{0.f, 0.f, 0.f, 0.f};
+      // the border index k = NC of the core outputs: H[i][NC] S[NC][j] - S[i][NC] conj(H[j][NC])
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const float2 sj = srow[J0 + 16 * u + lr], hj = hcol[J0 + 16 * u + lr];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = I0 + 4 * lg + v;
+          const float2 t = border_term(hcol[i], sj, scol[i], hj);
+          rp[u][v] += t.x;
+          ip[u][v] += t.y;
+        }
+      }
+#pragma unroll
+      for (int k0 = 0; k0 < KP; k0 += 32) {
+        // H S: A = H rows of tile row ti (shared), B = S columns of the wave's two tiles
+        {
+          const u16x8 hxh = frag(sH, aoff + k0), hxl = frag(sH + PLANE, aoff + k0);
+          const u16x8 hyh = frag(sH + 2 * PLANE, aoff + k0), hyl = frag(sH + 3 * PLANE, aoff + k0);
+          u16x8 sxh[2], sxl[2], syh[2], syl[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int o = boff + 16 * u * PITCH + k0;
+            sxh[u] = frag(sC, o);
+            sxl[u] = frag(sC + PLANE, o);
+            syh[u] = frag(sC + 2 * PLANE, o);
+            syl[u] = frag(sC + 3 * PLANE, o);
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            rp[u] = mfma3(hxh, hxl, sxh[u], sxl[u], rp[u]);   // Hx Sx
+            ip[u] = mfma3(hxh, hxl, syh[u], syl[u], ip[u]);   // Hx Sy
+            rn[u] = mfma3(hyh, hyl, syh[u], syl[u], rn[u]);   // Hy Sy
+            ip[u] = mfma3(hyh, hyl, sxh[u], sxl[u], ip[u]);   // Hy Sx
+          }
+          if (rowsk) {
+            const u16x8 k0h = frag(sK, koff + k0), k0l = frag(sK, koff + 2 * PITCH + k0);
+            const u16x8 k1h = frag(sK, koff + 4 * PITCH + k0), k1l = frag(sK, koff + 6 * PITCH + k0);
+            if (usk == 0) {
+              q = mfma3(k0h, k0l, sxh[0], sxl[0], q);
+              q = mfma3(k1h, k1l, syh[0], syl[0], q);
+            } else {
+              q = mfma3(k0h, k0l, sxh[1], sxl[1], q);
+              q = mfma3(k1h, k1l, syh[1], syl[1], q);
+            }
+          } else {
+            q = mfma3(hxh, hxl, frag(sK, koff + 16 * PITCH + k0), frag(sK, koff + 18 * PITCH + k0), q);
+            q = mfma3(hyh, hyl, frag(sK, koff + 20 * PITCH + k0), frag(sK, koff + 22 * PITCH + k0), q);
+          }
+        }
+        // S H^dag: A = S rows of tile row ti (shared), B[k][j] = conj(H[j][k]) from H rows of
+        // the wave's tile columns
+        {
+          const u16x8 zxh = frag(sR, aoff + k0), zxl = frag(sR + PLANE, aoff + k0);
+          const u16x8 zyh = frag(sR + 2 * PLANE, aoff + k0), zyl = frag(sR + 3 * PLANE, aoff + k0);
+          u16x8 gxh[2], gxl[2], gyh[2], gyl[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int o = boff + 16 * u * PITCH + k0;
+            gxh[u] = frag(sH, o);
+            gxl[u] = frag(sH + PLANE, o);
+            gyh[u] = frag(sH + 2 * PLANE, o);
+            gyl[u] = frag(sH + 3 * PLANE, o);
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            rn[u] = mfma3(zxh, zxl, gxh[u], gxl[u], rn[u]);   // Sx Hx'
+            ip[u] = mfma3(zxh, zxl, gyh[u], gyl[u], ip[u]);   // Sx Hy'
+            rn[u] = mfma3(zyh, zyl, gyh[u], gyl[u], rn[u]);   // Sy Hy'
+            in[u] = mfma3(zyh, zyl, gxh[u], gxl[u], in[u]);   // Sy Hx'
+          }
+          if (rowsk) {
+            const u16x8 k2h = frag(sK, koff + 8 * PITCH + k0), k2l = frag(sK, koff + 10 * PITCH + k0);
+            const u16x8 k3h = frag(sK, koff + 12 * PITCH + k0), k3l = frag(sK, koff + 14 * PITCH + k0);
+            if (usk == 0) {
+              q = mfma3(k2h, k2l, gxh[0], gxl[0], q);
+              q = mfma3(k3h, k3l, gyh[0], gyl[0], q);
+            } else {
+              q = mfma3(k2h, k2l, gxh[1], gxl[1], q);
+              q = mfma3(k3h, k3l, gyh[1], gyl[1], q);
+            }
+          } else {
+            q = mfma3(zxh, zxl, frag(sK, koff + 24 * PITCH + k0), frag(sK, koff + 26 * PITCH + k0), q);
+            q = mfma3(zyh, zyl, frag(sK, koff + 28 * PITCH + k0), frag(sK, koff + 30 * PITCH + k0), q);
+          }
+        }
+      }
+      // C at the owned border outputs (before the border index term and D o S)
+      float bre[2] = {0.f, 0.f}, bim[2] = {0.f, 0.f};
+      if (rowsk) {   // lanes 0..15: Re C[NC][j] in row 0, Im in row 1
+        bre[0] = q[0];
+        bim[0] = q[1];
+      } else {
+        // lane lr = 0 holds Re C[I0 + 4 lg + v][NC] in q[v], lane lr = 1 the Im part; the
+        // even lane owns rows v = 0, 1, the odd lane rows v = 2, 3
+        const float o0 = dpp_f32<0xB1>(q[0]), o1 = dpp_f32<0xB1>(q[1]);   // lane l ^ 1
+        const float o2 = dpp_f32<0xB1>(q[2]), o3 = dpp_f32<0xB1>(q[3]);
+        const bool odd = (lr & 1) != 0;
+        bre[0] = odd ? o2 : q[0];
+        bim[0] = odd ? q[2] : o0;
+        bre[1] = odd ? o3 : q[1];
+        bim[1] = odd ? q[3] : o1;
+      }
+      if (w == L::CORNER_WAVE) {   // the corner over k < NC: one k per lane, then a wave sum
+        float qr = 0.f, qm = 0.f;
+        for (int k = lane; k < NC; k += 64) {
+          const float2 t = border_term(hrow[k], scol[k], srow[k], hrow[k]);
+          qr += t.x;
+          qm += t.y;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          qr += __shfl_xor(qr, off, 64);
+          qm += __shfl_xor(qm, off, 64);
+        }
+        if (own_corner) {
+          bre[0] = qr;
+          bim[0] = qm;
+        }
+      }
+      // the border index term and D o S of the border outputs: k_b = -i C_b + D o S
+      float2 kb[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        kb[u] = make_float2(0.f, 0.f);
+        if (u < nbo) {
+          const int r = bi(u), c = bj(u);
+          const float2 t = border_term(hcol[r], srow[c], scol[r], hcol[c]);
+          const float2 sv = r == NC ? srow[c] : scol[r];
+          const float2 p = cmul(bD[bslot(u)], sv);
+          kb[u] = make_float2(bim[u] + t.y + p.x, -(bre[u] + t.x) + p.y);
+        }
+      }
+      __syncthreads();   // every wave has finished reading S
+
+      const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
+      const float keep = (st == 0) ? 0.f : 1.f;   // the increment restarts at stage 0
+      float2 sv[2][4], svb[2];
+      if (st < 3) {
+        // next stage operand S = rho + c k (FP64, rounded to FP32, then split)
+        const double cnext = (st == 2) ? dt : 0.5 * dt;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float kr = ip[u][v] - in[u][v], ki = -(rp[u][v] - rn[u][v]);
+            inc[u][v].x = fmaf(cacc, kr, keep * inc[u][v].x);
+            inc[u][v].y = fmaf(cacc, ki, keep * inc[u][v].y);
+            sv[u][v] = make_float2((float)(rho[u][v].x + cnext * (double)kr),
+                                   (float)(rho[u][v].y + cnext * (double)ki));
+          }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          svb[u] = make_float2(0.f, 0.f);
+          if (u < nbo) {
+            const int x = bslot(u);
+            const double2 r = bRho[x];
+            float2 ib = bInc[x];
+            ib.x = fmaf(cacc, kb[u].x, keep * ib.x);
+            ib.y = fmaf(cacc, kb[u].y, keep * ib.y);
+            bInc[x] = ib;
+            svb[u] = make_float2((float)(r.x + cnext * (double)kb[u].x), (float)(r.y + cnext * (double)kb[u].y));
+          }
+        }
+      } else {
+        // last stage: the state update itself is FP64
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float kr = ip[u][v] - in[u][v], ki = -(rp[u][v] - rn[u][v]);
+            rho[u][v].x += (double)fmaf(cacc, kr, inc[u][v].x);
+            rho[u][v].y += (double)fmaf(cacc, ki, inc[u][v].y);
+          }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          if (u < nbo) {
+            const int x = bslot(u);
+            double2 r = bRho[x];
+            const float2 ib = bInc[x];
+            r.x += (double)fmaf(cacc, kb[u].x, ib.x);
+            r.y += (double)fmaf(cacc, kb[u].y, ib.y);
+            bRho[x] = r;
+          }
+        }
+        if (renorm) {
+          double tr = 0.0;
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+            if (J0 + 16 * u == I0) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                if (4 * lg + v == lr) tr += rho[u][v].x;
+            }
+          if (own_corner) tr += bRho[2 * NC].x;
+          tr = wave_sum(tr);
+          if (lane == 0) sRed[w].x = tr;
+          __syncthreads();
+          double tot = 0.0;
+          for (int x = 0; x < W; ++x) tot += sRed[x].x;
+          const double inv = 1.0 / tot;
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              rho[u][v].x *= inv;
+              rho[u][v].y *= inv;
+            }
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            if (u < nbo) {
+              double2 r = bRho[bslot(u)];
+              r.x *= inv;
+              r.y *= inv;
+              bRho[bslot(u)] = r;
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sv[u][v] = make_float2((float)rho[u][v].x, (float)rho[u][v].y);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const double2 r = u < nbo ? bRho[bslot(u)] : make_double2(0.0, 0.0);
+          svb[u] = make_float2((float)r.x, (float)r.y);
+        }
+      }
+
+      // ---- write the next stage operand: core elements in both layouts, border outputs
+      // into the side arrays and the coefficient operands; D o S into the accumulators
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        unsigned xh0, xl0, yh0, yl0, xh1, xl1, yh1, yl1;
+        split2(sv[u][0].x, sv[u][1].x, xh0, xl0);
+        split2(sv[u][0].y, sv[u][1].y, yh0, yl0);
+        split2(sv[u][2].x, sv[u][3].x, xh1, xl1);
+        split2(sv[u][2].y, sv[u][3].y, yh1, yl1);
+        const int j = J0 + 16 * u + lr;
+        u16* c0 = sC + j * PITCH + I0 + 4 * lg;   // column plane row j, positions i..i+3
+        *reinterpret_cast<uint2*>(c0) = make_uint2(xh0, xh1);
+        *reinterpret_cast<uint2*>(c0 + PLANE) = make_uint2(xl0, xl1);
+        *reinterpret_cast<uint2*>(c0 + 2 * PLANE) = make_uint2(yh0, yh1);
+        *reinterpret_cast<uint2*>(c0 + 3 * PLANE) = make_uint2(yl0, yl1);
+        u16* r0 = sR + (I0 + 4 * lg) * PITCH + j;   // row plane rows i..i+3, position j
+        const unsigned pl[4][2] = {{xh0, xh1}, {xl0, xl1}, {yh0, yh1}, {yl0, yl1}};
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          r0[p * PLANE] = (u16)pl[p][0];
+          r0[p * PLANE + PITCH] = (u16)(pl[p][0] >> 16);
+          r0[p * PLANE + 2 * PITCH] = (u16)pl[p][1];
+          r0[p * PLANE + 3 * PITCH] = (u16)(pl[p][1] >> 16);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float2 p = cmul(dc[u][v], sv[u][v]);
+          rp[u][v] = -p.y;
+          ip[u][v] = p.x;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (u < nbo) {
+          const int r = bi(u), c = bj(u);
+          if (r == NC) {
+            srow[c] = svb[u];
+            if (c < NC) put_coef_srow<PITCH>(sK, c, svb[u]);
+          }
+          if (c == NC) {
+            scol[r] = svb[u];
+            if (r < NC) put_coef_scol<PITCH>(sK, r, svb[u]);
+          }
+        }
+      }
+      if (st < 3) {
+        __syncthreads();   // S of the next stage is complete
+        continue;
+      }
+
+      // ---------------- end of step: records
+      const int g = a.step_offset + s + 1;
+      const bool rec_now = record_any && (g % a.nout) == 0 && (g / a.nout - 1) < a.rec_capacity;
+      if (rec_now) {
+        const int rec = g / a.nout - 1;
+        int olr = lr, olg = lg;
+        asm volatile("" : "+v"(olr), "+v"(olg));
+        if (a.obs != nullptr) {
+          const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
+          for (int e = 0; e < a.n_eops; ++e) {
+            const double2* E = E0 + (size_t)e * nn;
+            double pr = 0.0, pi = 0.0;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              double2 ev[4];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) ev[v] = E[(I0 + v) * n + J0 + 16 * u + 4 * olg * n + olr];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                pr += ev[v].x * rho[u][v].x - ev[v].y * rho[u][v].y;
+                pi += ev[v].x * rho[u][v].y + ev[v].y * rho[u][v].x;
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              if (u < nbo) {
+                const double2 x = E[bi(u) * n + bj(u)];
+                const double2 r = bRho[bslot(u)];
+                pr += x.x * r.x - x.y * r.y;
+                pi += x.x * r.y + x.y * r.x;
+              }
+            }
+            pr = wave_sum(pr);
+            pi = wave_sum(pi);
+            // slots W.. are free: slots 0..W-1 may still be read by the trace step
+            if (lane == 0) sRed[W + e * W + w] = make_double2(pr, pi);
+          }
+        }
+        if (a.states != nullptr) {
+          double2* Sg = a.states + ((size_t)b * a.rec_capacity + rec) * nn;
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Sg[(I0 + v) * n + J0 + 16 * u + 4 * olg * n + olr] = rho[u][v];
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+            if (u < nbo) Sg[bi(u) * n + bj(u)] = bRho[bslot(u)];
+        }
+      }
+      __syncthreads();   // S of the next step (and the record partials) are complete
+      if (rec_now && a.obs != nullptr && w == W - 1) {
+        const int rec = g / a.nout - 1;
+        int e0 = lane;
+        asm volatile("" : "+v"(e0));
+        for (int e = e0; e < a.n_eops; e += 64) {
+          double2 t = make_double2(0.0, 0.0);
+          for (int x = 0; x < W; ++x) {
+            const double2 y = sRed[W + e * W + x];
+            t.x += y.x;
+            t.y += y.y;
+          }
+          a.obs[((size_t)b * a.rec_capacity + rec) * a.n_eops + e] = t;
+        }
+      }
+    }
+  }
+
+  // ---- write the state back
+  {
+    double2* R = a.rho + (size_t)b * nn;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) R[(I0 + 4 * lg + v) * n + J0 + 16 * u + lr] = rho[u][v];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (u < nbo) R[bi(u) * n + bj(u)] = bRho[bslot(u)];
+  }
+}
+
+namespace {
+
+template <bool REC>
+hipError_t launch_bf3_one(const ResidentArgs& a, size_t lds, hipStream_t stream) {
+  auto k = lindblad_rk4_resident_bf3<REC>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(a.batch), dim3(Bf3Layout::W * 64), lds, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool resident_bf3_eligible(int n, int n_dense) { return n_dense == 0 && n == 65; }
+
+size_t resident_bf3_lds_bytes(int n, int n_eops) {
+  if (n != 65) return 0;
+  return Bf3Layout::kFixedBytes + (size_t)(1 + n_eops) * Bf3Layout::W * sizeof(double2);
+}
+
+hipError_t resident_bf3_launch(const ResidentArgs& a, hipStream_t stream) {
+  if (!resident_bf3_eligible(a.n, a.n_dense) || a.rops != nullptr) return hipErrorInvalidValue;
+  const size_t lds = resident_bf3_lds_bytes(a.n, a.n_eops);
+  const bool rec = a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
+  return rec ? launch_bf3_one<true>(a, lds, stream) : launch_bf3_one<false>(a, lds, stream);
+}
+
+}  // namespace lime

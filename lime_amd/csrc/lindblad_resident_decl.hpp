@@ -30,4 +30,10 @@ size_t resident_lds_bytes(int n, int variant, int n_dense, int n_eops);
 bool resident_double_buffered(int n, int variant, int n_dense, int n_eops);
 hipError_t resident_launch(const ResidentArgs& a, int variant, hipStream_t stream);
 
+// 3xBF16 split-product resident kernel (lindblad_resident_bf3.hip): bordered n = 33, 49, 65
+// without dense operators; one launch shape
+bool resident_bf3_eligible(int n, int n_dense);
+size_t resident_bf3_lds_bytes(int n, int n_eops);
+hipError_t resident_bf3_launch(const ResidentArgs& a, hipStream_t stream);
+
 }  // namespace lime
