@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU comparator")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy checks")
+    ap.add_argument("--no-bf3", action="store_true",
+                    help="skip the 3xBF16 resident-kernel leg of the default workload")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense-operand leg of the default workload")
     ap.add_argument("--workload", default="lattice65",
@@ -241,14 +243,22 @@ def main():
     peak = float(torch.max(spec.real).item())
 
     extra = {}
+    refs = qrt_references(cfg, min(cfg["nt"], total_steps), R) if not args.no_check else None
     if not args.no_check:
-        extra.update(accuracy_checks(cfg, prop, obs, total_steps, dt, R, prec))
+        extra.update(accuracy_checks(cfg, obs, total_steps, dt, R, refs, prec))
+    if prec == "fp32" and not args.no_bf3:
+        X0 = torch.as_tensor(cfg["x0"].reshape(B, n, n), device=dev).contiguous()
+        b3 = bf3_leg(cfg, X0, E_t, args.steps, chunk, refs, check=not args.no_check)
+        extra["bf16x3_resident"] = b3
     if not args.no_dense:
         d = dense_operand_leg(cfg, prop.variant, 3, prec)
         extra["dense_operand"] = d
         extra["dense_operand_rho_steps_per_s"] = d["value"]
     if not args.no_plain:
         extra.update(plain_fp64(cfg, B, dt, value))
+        if "bf16x3_resident" in extra:
+            b3 = extra["bf16x3_resident"]
+            b3["speedup_vs_plain_fp64"] = b3["value"] / extra["plain_fp64_gpu"]["value"]
 
     traffic, traffic_note = profiled_traffic(args, prop.variant)
     if prec == "3xbf16":
@@ -309,58 +319,109 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def accuracy_checks(cfg, prop, obs, total_steps, dt, R, prec="fp32"):
-    """Max errors vs the FP64 oracle on the records of the timed run itself."""
-    import torch
+def qrt_references(cfg, nchk, R):
+    """FP64 oracle C_a(t) of the first and the last realisation (three components each)
+    over nchk steps: {r: (3, nchk) complex}."""
     from oracle import lindblad as ol
+    refs = {}
+    for r in (0, R - 1):
+        refs[r] = np.stack([ol.propagate(cfg["H"][r], cfg["x0"][r, a], cfg["c_ops"], cfg["dt"], nchk,
+                                         [cfg["mu"][r, a]])[1][:, 0] for a in range(3)])
+    return refs
+
+
+def qrt_errors(obs, refs, dt, nchk):
+    """Max C(t) error of a run's records against the oracle (relative to max |C|), and the
+    max error of each realisation's spectrum (lime's transform, fft.py:9-56: rocFFT on the
+    GPU records against numpy FFT on the oracle's) relative to the spectrum's peak."""
     from oracle import spectra as osp
     from lime_amd import fft as gfft
-    from lime_amd.propagator import LindbladPropagator
-    out = {}
-    nt = cfg["nt"]
-    nchk = min(nt, total_steps)
-    # (1) C_a(t) of the first and the last realisation, all three components, over the whole
-    #     nt-step trajectory of the workload, from the timed run's own records
-    errs, scale, spec_err, spec_scale = [], 0.0, 0.0, 0.0
     t = dt * np.arange(1, nchk + 1)
-    for r in (0, R - 1):
-        iso_ref = np.zeros(nchk, dtype=np.complex128)
-        for a in range(3):
-            _, ref, _ = ol.propagate(cfg["H"][r], cfg["x0"][r, a], cfg["c_ops"], dt, nchk,
-                                     [cfg["mu"][r, a]])
-            got = obs[3 * r + a, :nchk, 0].cpu().numpy()
-            errs.append(np.max(np.abs(got - ref[:, 0])))
-            scale = max(scale, np.max(np.abs(ref)))
-            iso_ref += ref[:, 0] / 3.0
-        # (2) the realisation's spectrum: lime's transform (fft.py:9-56) of its isotropic
-        #     C(t), rocFFT on the GPU records against numpy FFT on the oracle's
-        iso_gpu = obs[3 * r:3 * r + 3, :nchk, 0].sum(dim=0) / 3.0
-        s_gpu, _ = gfft.fft(iso_gpu, t)
-        s_ref, _ = osp.fft(iso_ref, t)
+    err, scale, spec_err, spec_scale = 0.0, 0.0, 0.0, 0.0
+    for r, ref in refs.items():
+        got = obs[3 * r:3 * r + 3, :nchk, 0]
+        err = max(err, float(np.max(np.abs(got.cpu().numpy() - ref))))
+        scale = max(scale, float(np.max(np.abs(ref))))
+        s_gpu, _ = gfft.fft(got.sum(dim=0) / 3.0, t)
+        s_ref, _ = osp.fft(ref.sum(axis=0) / 3.0, t)
         spec_err = max(spec_err, float(np.max(np.abs(s_gpu.cpu().numpy() - s_ref))))
         spec_scale = max(spec_scale, float(np.max(np.abs(s_ref))))
-    out["qrt_max_abs_err"] = float(max(errs))
-    out["qrt_max_rel_err"] = float(max(errs) / scale)
-    out["qrt_checked"] = f"realisations 0 and {R - 1}, components x/y/z, records 1..{nchk}"
-    out["spectrum_max_rel_err"] = spec_err / spec_scale
-    log(f"[check] C(t) max error {max(errs):.3e} (rel {max(errs) / scale:.3e}) over {nchk} steps, "
-        f"realisations 0 and {R - 1}; spectrum rel error {spec_err / spec_scale:.3e}")
-    # (3) density-matrix populations over the full trajectory for realisation 0:
-    #     rho0 = |mu_x><mu_x| / <mu_x|mu_x> in the site manifold (impulsive excitation)
-    n = cfg["n"]
+    return err, err / scale, spec_err / spec_scale
+
+
+def population_check(cfg, dt, prec):
+    """Density-matrix populations of realisation 0 over the workload's nt steps against the
+    oracle: rho0 = |mu_x><mu_x| / <mu_x|mu_x> in the site manifold (impulsive excitation)."""
+    from oracle import lindblad as ol
+    from lime_amd.propagator import LindbladPropagator
+    n, nt = cfg["n"], cfg["nt"]
     v = cfg["mu"][0, 0][:, 0].copy()
     rho0 = np.outer(v, v.conj()) / np.vdot(v, v)
     P = [np.diag(np.eye(n)[i]).astype(np.complex128) for i in range(n)]
     p1 = LindbladPropagator(cfg["H"][0], cfg["c_ops"], precision=prec)
     rho, pobs, _ = p1.run(rho0, dt, nt, e_ops=P, nout=16)
     _, ref, _ = ol.propagate(cfg["H"][0], rho0, cfg["c_ops"], dt, nt, P, nout=16)
-    pe = float(np.max(np.abs(pobs[0].cpu().numpy() - ref)))
     r = rho[0].cpu().numpy()
-    out["pop_max_abs_err"] = pe
-    out["trace_residual"] = float(abs(np.trace(r) - 1))
-    out["hermiticity_residual"] = float(np.max(np.abs(r - r.conj().T)))
-    log(f"[check] populations over {nt} steps: max error {pe:.3e}; "
+    return {"pop_max_abs_err": float(np.max(np.abs(pobs[0].cpu().numpy() - ref))),
+            "trace_residual": float(abs(np.trace(r) - 1)),
+            "hermiticity_residual": float(np.max(np.abs(r - r.conj().T)))}
+
+
+def accuracy_checks(cfg, obs, total_steps, dt, R, refs, prec="fp32"):
+    """Max errors vs the FP64 oracle on the records of the timed run itself."""
+    out = {}
+    nchk = min(cfg["nt"], total_steps)
+    # (1) C_a(t) of the first and the last realisation, all three components, over the whole
+    #     nt-step trajectory of the workload, from the timed run's own records; (2) their spectra
+    err, rel, spec = qrt_errors(obs, refs, dt, nchk)
+    out["qrt_max_abs_err"] = err
+    out["qrt_max_rel_err"] = rel
+    out["qrt_checked"] = f"realisations 0 and {R - 1}, components x/y/z, records 1..{nchk}"
+    out["spectrum_max_rel_err"] = spec
+    log(f"[check] C(t) max error {err:.3e} (rel {rel:.3e}) over {nchk} steps, "
+        f"realisations 0 and {R - 1}; spectrum rel error {spec:.3e}")
+    # (3) density-matrix populations over the full trajectory for realisation 0
+    out.update(population_check(cfg, dt, prec))
+    log(f"[check] populations over {cfg['nt']} steps: max error {out['pop_max_abs_err']:.3e}; "
         f"trace residual {out['trace_residual']:.2e}; hermiticity {out['hermiticity_residual']:.2e}")
+    return out
+
+
+def bf3_leg(cfg, X0, E_t, launches, chunk, refs, check=True):
+    """The same workload on the 3xBF16 resident kernel (split BF16 products on MFMA, FP32
+    accumulation, FP64 state): one warm-up launch, then `launches` timed launches covering
+    the trajectory; C(t), spectrum and population errors against the FP64 oracle."""
+    import torch
+    from lime_amd.propagator import LindbladPropagator
+    R, n, dt = cfg["H"].shape[0], cfg["n"], cfg["dt"]
+    B = 3 * R
+    prop = LindbladPropagator(cfg["H"], cfg["c_ops"], group=3, precision="3xbf16")
+    X = X0.clone()
+    total = (launches + 1) * chunk
+    obs = torch.zeros((B, total, 1), dtype=torch.complex128, device=X.device)
+    step = lambda i: prop.step(X, dt, chunk, eops_t=E_t, eop_group=1, obs=obs, nout=1,
+                               step_offset=i * chunk, rec_capacity=total)
+    step(0)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
+    ev[0].record()
+    for i in range(launches):
+        step(1 + i)
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[-1]) / launches
+    value = B * chunk / (ms / 1e3)
+    tflops = flops_per_rho_step(n) * B * chunk / (ms / 1e3) / 1e12
+    out = {"value": value, "unit": "rho-steps/s", "tflops_canonical": tflops,
+           "frac_of_fp32_peak": tflops / PEAK_FP32_MFMA_TFLOPS, "launch_ms_mean": ms,
+           "kernel": "lindblad_rk4_resident_bf3 (n = 65, 8 waves x 2 tiles, 3xBF16 split products)"}
+    log(f"[bf3] 3xBF16 resident: {value:.4g} rho-steps/s, {tflops:.2f} TFLOP/s canonical ({ms:.1f} ms per launch)")
+    if check:
+        nchk = min(cfg["nt"], total)
+        err, rel, spec = qrt_errors(obs, refs, dt, nchk)
+        out.update({"qrt_max_rel_err": rel, "spectrum_max_rel_err": spec})
+        out.update(population_check(cfg, dt, "3xbf16"))
+        log(f"[bf3] C(t) rel err {rel:.3e}, spectrum {spec:.3e}, populations {out['pop_max_abs_err']:.3e}")
     return out
 
 

@@ -11,10 +11,15 @@
 //
 // Work decomposition (n = NC + 1, NC = 64; 8 waves, two 16 x 16 core tiles each):
 //   * LDS holds the split operands as BF16 planes, one per part (Re hi, Re lo, Im hi, Im lo):
-//     H_eff by rows (A operand of H S; B operand of S H^dag, whose B[k][j] = conj(H[j][k]));
-//     S by rows (A of S H^dag) and by columns (B of H S).  An MFMA operand fragment is 8
-//     consecutive k of one plane row: one 16-byte LDS read.  The lanes that own S elements
-//     write both layouts in the epilogue of every stage.
+//     H_eff by rows (A operand of H S; B operand of S H^dag, whose B[k][j] = conj(H[j][k]))
+//     and the stage operand S by columns (plane row j holds S[k][j]: B operand of H S, one
+//     16-byte read of 8 consecutive k).  The A operand of S H^dag (S rows) comes from the
+//     same column planes through ds_read_b64_tr_b16, whose result lane i, element e is
+//     element i % 4 of what lane 4 e + i / 4 of its 16-lane group loaded: lane 4 e + q
+//     loads S[4 q .. 4 q + 3][k + e] (8 bytes of plane row k + e), so lane i receives
+//     S[i][k .. k + 3].  The lanes that own S elements write 4 consecutive rows of one
+//     column per plane in the epilogue (one 8-byte store each), into the other of two
+//     buffers: one barrier per stage.
 //   * C = H S - S H^dag over k < NC on MFMA, 4 real products per complex product into split-
 //     sign FP32 accumulators (no negated operand in the MFMA stream); the border index
 //     k = NC is an FP32 VALU term from side arrays (row and column NC of H and S).
@@ -141,11 +146,43 @@ struct Bf3Layout {
   static constexpr int PLANE = NC * PITCH;
   static constexpr int NCP = NC + 2;                 // side-array length (keeps 16-byte alignment)
   static constexpr int NBO = 2 * NC + 2;             // border-output slots (2 NC + 1 used)
-  static constexpr size_t kBf16Bytes = (size_t)(12 * PLANE + 32 * PITCH) * 2;
-  // + side arrays (4 x NCP complex64) + border-output state (FP64 rho, FP32 increment, D)
-  static constexpr size_t kFixedBytes = kBf16Bytes + (size_t)4 * NCP * 8 + (size_t)NBO * 32;
+  // one stage buffer: S column planes, the coefficient operands (32 rows), S[NC][:], S[:][NC]
+  static constexpr size_t kStageBytes = (size_t)(4 * PLANE + 32 * PITCH) * 2 + (size_t)2 * NCP * 8;
+  static constexpr size_t kHBytes = (size_t)4 * PLANE * 2;   // H_eff row planes
+  // H_eff planes, two stage buffers, H[:][NC] and H[NC][:], border-output state (FP64 rho,
+  // FP32 increment, D)
+  static constexpr size_t kFixedBytes = kHBytes + 2 * kStageBytes + (size_t)2 * NCP * 8 + (size_t)NBO * 32;
   static constexpr int CORNER_WAVE = 1, CORNER_LANE = 2;
 };
+
+struct Bf3Stage {   // pointers into one stage buffer
+  u16* sC;
+  u16* sK;
+  float2* srow;
+  float2* scol;
+};
+
+__device__ __forceinline__ Bf3Stage bf3_stage(unsigned char* base) {
+  using L = Bf3Layout;
+  Bf3Stage b;
+  b.sC = reinterpret_cast<u16*>(base);
+  b.sK = b.sC + 4 * L::PLANE;
+  b.srow = reinterpret_cast<float2*>(b.sK + 32 * L::PITCH);
+  b.scol = b.srow + L::NCP;
+  return b;
+}
+
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) short4v lds_short4;
+
+// A-operand fragment of S rows (rows I0 + lr, k = k0 + 8 lg .. + 7) from a column plane:
+// two transposed 8-byte reads, lane (lg, lr) loading plane rows k0 + 8 lg + lr / 4 (+ 4),
+// positions I0 + 4 (lr % 4) .. + 3
+__device__ __forceinline__ u16x8 frag_rows(const u16* plane, int off) {
+  const short4v a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(plane + off));
+  const short4v b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(plane + off + 4 * Bf3Layout::PITCH));
+  return u16x8{(u16)a[0], (u16)a[1], (u16)a[2], (u16)a[3], (u16)b[0], (u16)b[1], (u16)b[2], (u16)b[3]};
+}
 
 }  // namespace
 
@@ -158,16 +195,12 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
 
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   u16* sH = reinterpret_cast<u16*>(lds_raw);   // H_eff rows: planes Re hi, Re lo, Im hi, Im lo
-  u16* sR = sH + 4 * PLANE;                    // S rows
-  u16* sC = sR + 4 * PLANE;                    // S columns: plane row j holds S[k][j]
-  u16* sK = sC + 4 * PLANE;                    // border coefficient operands (32 rows)
-  float2* hcol = reinterpret_cast<float2*>(lds_raw + L::kBf16Bytes);   // H[r][NC]
-  float2* hrow = hcol + L::NCP;                                         // H[NC][k]
-  float2* srow = hrow + L::NCP;                                         // S[NC][k]
-  float2* scol = srow + L::NCP;                                         // S[r][NC]
+  auto stage_buf = [&](int x) { return bf3_stage(lds_raw + L::kHBytes + x * L::kStageBytes); };
+  float2* hcol = reinterpret_cast<float2*>(lds_raw + L::kHBytes + 2 * L::kStageBytes);   // H[r][NC]
+  float2* hrow = hcol + L::NCP;                                                          // H[NC][k]
   // border outputs' state, one slot each (row output j: slot j; column output i: NC + i;
   // the corner: 2 NC), read and written only by the owning lane
-  double2* bRho = reinterpret_cast<double2*>(scol + L::NCP);
+  double2* bRho = reinterpret_cast<double2*>(hrow + L::NCP);
   float2* bInc = reinterpret_cast<float2*>(bRho + L::NBO);
   float2* bD = bInc + L::NBO;
   double2* sRed = reinterpret_cast<double2*>(lds_raw + L::kFixedBytes);
@@ -185,18 +218,18 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
   const int ti = w >> 1, c2 = w & 1;           // tile row, column pair (wave-uniform)
   const int I0 = 16 * ti, J0 = 32 * c2;        // the wave's tiles: columns J0 .. J0 + 31
   const bool rowsk = c2 == (ti >> 1);          // row outputs of tile column ti (tile u = ti & 1)
-  const bool colsk = !rowsk;                   // column outputs of tile row ti
   const int usk = ti & 1;
 
-  // ---- LDS: zero the BF16 region (k >= NC stays zero), then fill it
+  // ---- LDS: zero the planes and coefficient rows (k >= NC stays zero), then fill them
   {
     uint4* z = reinterpret_cast<uint4*>(lds_raw);
-    for (int i = tid; i < (int)(L::kBf16Bytes / 16); i += T) z[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = tid; i < (int)((L::kHBytes + 2 * L::kStageBytes) / 16); i += T) z[i] = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
   {
     const float2* H = a.heff + (size_t)set * nn;
     const double2* R = a.rho + (size_t)b * nn;
+    const Bf3Stage s0 = stage_buf(0), s1 = stage_buf(1);
     for (int idx = tid; idx < NC * (NC / 2); idx += T) {   // element pairs (i, k), (i, k + 1)
       const int i = idx / (NC / 2), k = 2 * (idx - i * (NC / 2));
       unsigned xh, xl, yh, yl;
@@ -208,24 +241,15 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       *reinterpret_cast<unsigned*>(hr + PLANE) = xl;
       *reinterpret_cast<unsigned*>(hr + 2 * PLANE) = yh;
       *reinterpret_cast<unsigned*>(hr + 3 * PLANE) = yl;
-      const double2 r0 = R[i * n + k], r1 = R[i * n + k + 1];
+      // S[k][i], S[k + 1][i]: column plane row i, positions k, k + 1
+      const double2 r0 = R[k * n + i], r1 = R[(k + 1) * n + i];
       split2((float)r0.x, (float)r1.x, xh, xl);
       split2((float)r0.y, (float)r1.y, yh, yl);
-      u16* rr = sR + i * PITCH + k;
-      *reinterpret_cast<unsigned*>(rr) = xh;
-      *reinterpret_cast<unsigned*>(rr + PLANE) = xl;
-      *reinterpret_cast<unsigned*>(rr + 2 * PLANE) = yh;
-      *reinterpret_cast<unsigned*>(rr + 3 * PLANE) = yl;
-      u16* c0 = sC + k * PITCH + i;   // S[i][k] -> column plane row k, position i
-      u16* c1 = c0 + PITCH;           // S[i][k + 1]
-      c0[0] = (u16)xh;
-      c1[0] = (u16)(xh >> 16);
-      c0[PLANE] = (u16)xl;
-      c1[PLANE] = (u16)(xl >> 16);
-      c0[2 * PLANE] = (u16)yh;
-      c1[2 * PLANE] = (u16)(yh >> 16);
-      c0[3 * PLANE] = (u16)yl;
-      c1[3 * PLANE] = (u16)(yl >> 16);
+      u16* cr = s0.sC + i * PITCH + k;
+      *reinterpret_cast<unsigned*>(cr) = xh;
+      *reinterpret_cast<unsigned*>(cr + PLANE) = xl;
+      *reinterpret_cast<unsigned*>(cr + 2 * PLANE) = yh;
+      *reinterpret_cast<unsigned*>(cr + 3 * PLANE) = yl;
     }
     for (int k = tid; k <= NC; k += T) {
       const float2 hc = H[k * n + NC], hr = H[NC * n + k];
@@ -233,12 +257,13 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       const float2 sr = make_float2((float)rr.x, (float)rr.y), sc = make_float2((float)rc.x, (float)rc.y);
       hcol[k] = hc;
       hrow[k] = hr;
-      srow[k] = sr;
-      scol[k] = sc;
+      s0.srow[k] = sr;
+      s0.scol[k] = sc;
       if (k < NC) {
-        put_coef_h<PITCH>(sK, k, hr);
-        put_coef_srow<PITCH>(sK, k, sr);
-        put_coef_scol<PITCH>(sK, k, sc);
+        put_coef_h<PITCH>(s0.sK, k, hr);   // the constant operands, in both buffers
+        put_coef_h<PITCH>(s1.sK, k, hr);
+        put_coef_srow<PITCH>(s0.sK, k, sr);
+        put_coef_scol<PITCH>(s0.sK, k, sc);
       }
     }
   }
@@ -246,7 +271,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
   // ---- border outputs owned by this lane (row outputs: lanes 0..15; column outputs: lanes
   // with lr < 2, rows 4 lg + 2 lr + u of tile row ti; the corner)
   const bool own_row = rowsk && lane < 16;
-  const bool own_col = colsk && lr < 2;
+  const bool own_col = !rowsk && lr < 2;
   const bool own_corner = w == L::CORNER_WAVE && lane == L::CORNER_LANE;
   const int nbo = own_col ? 2 : (own_row || own_corner) ? 1 : 0;
   auto bi = [&](int u) { return own_col ? I0 + 4 * lg + 2 * lr + u : NC; };
@@ -297,15 +322,22 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
   const double dt = a.dt;
   const bool renorm = (a.flags & 1) != 0;
   const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
-  const int aoff = (I0 + lr) * PITCH + 8 * lg;   // fragments of tile row ti (A operands)
-  const int boff = (J0 + lr) * PITCH + 8 * lg;   // fragments of the wave's first tile column (B)
+  const int aoff = (I0 + lr) * PITCH + 8 * lg;                    // H rows of tile row ti (A)
+  const int boff = (J0 + lr) * PITCH + 8 * lg;                    // plane rows of the wave's first tile column (B)
+  const int troff = (8 * lg + (lr >> 2)) * PITCH + I0 + 4 * (lr & 3);   // S rows of tile row ti (A, transposed reads)
   const int koff = (lr & 1) * PITCH + 8 * lg;    // coefficient rows (rows 2..15 repeat 0, 1:
                                                  // their outputs are not used)
   auto frag = [&](const u16* base, int off) { return *reinterpret_cast<const u16x8*>(base + off); };
 
+  int ph = 0;   // buffer holding the current stage operand
   for (int s = 0; s < a.nsteps; ++s) {
 #pragma unroll 1
     for (int st = 0; st < 4; ++st) {
+      const Bf3Stage cur = stage_buf(ph), nxt = stage_buf(ph ^ 1);
+      const u16* sC = cur.sC;
+      const u16* sK = cur.sK;
+      const float2* srow = cur.srow;
+      const float2* scol = cur.scol;
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         rn[u] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -361,11 +393,12 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             q = mfma3(hyh, hyl, frag(sK, koff + 20 * PITCH + k0), frag(sK, koff + 22 * PITCH + k0), q);
           }
         }
-        // S H^dag: A = S rows of tile row ti (shared), B[k][j] = conj(H[j][k]) from H rows of
-        // the wave's tile columns
+        // S H^dag: A = S rows of tile row ti (shared; transposed reads of the column planes),
+        // B[k][j] = conj(H[j][k]) from H rows of the wave's tile columns
         {
-          const u16x8 zxh = frag(sR, aoff + k0), zxl = frag(sR + PLANE, aoff + k0);
-          const u16x8 zyh = frag(sR + 2 * PLANE, aoff + k0), zyl = frag(sR + 3 * PLANE, aoff + k0);
+          const int to = troff + k0 * PITCH;
+          const u16x8 zxh = frag_rows(sC, to), zxl = frag_rows(sC + PLANE, to);
+          const u16x8 zyh = frag_rows(sC + 2 * PLANE, to), zyl = frag_rows(sC + 3 * PLANE, to);
           u16x8 gxh[2], gxl[2], gyh[2], gyl[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -444,7 +477,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
           kb[u] = make_float2(bim[u] + t.y + p.x, -(bre[u] + t.x) + p.y);
         }
       }
-      __syncthreads();   // every wave has finished reading S
+      // (no barrier: the next stage operand goes to the other buffer)
 
       const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
       const float keep = (st == 0) ? 0.f : 1.f;   // the increment restarts at stage 0
@@ -540,8 +573,9 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
         }
       }
 
-      // ---- write the next stage operand: core elements in both layouts, border outputs
-      // into the side arrays and the coefficient operands; D o S into the accumulators
+      // ---- write the next stage operand into the other buffer: core elements (4
+      // consecutive rows of one column per plane), border outputs into the side arrays and
+      // the coefficient operands; D o S into the accumulators
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         unsigned xh0, xl0, yh0, yl0, xh1, xl1, yh1, yl1;
@@ -549,21 +583,11 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
         split2(sv[u][0].y, sv[u][1].y, yh0, yl0);
         split2(sv[u][2].x, sv[u][3].x, xh1, xl1);
         split2(sv[u][2].y, sv[u][3].y, yh1, yl1);
-        const int j = J0 + 16 * u + lr;
-        u16* c0 = sC + j * PITCH + I0 + 4 * lg;   // column plane row j, positions i..i+3
+        u16* c0 = nxt.sC + (J0 + 16 * u + lr) * PITCH + I0 + 4 * lg;   // column plane row j, positions i..i+3
         *reinterpret_cast<uint2*>(c0) = make_uint2(xh0, xh1);
         *reinterpret_cast<uint2*>(c0 + PLANE) = make_uint2(xl0, xl1);
         *reinterpret_cast<uint2*>(c0 + 2 * PLANE) = make_uint2(yh0, yh1);
         *reinterpret_cast<uint2*>(c0 + 3 * PLANE) = make_uint2(yl0, yl1);
-        u16* r0 = sR + (I0 + 4 * lg) * PITCH + j;   // row plane rows i..i+3, position j
-        const unsigned pl[4][2] = {{xh0, xh1}, {xl0, xl1}, {yh0, yh1}, {yl0, yl1}};
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          r0[p * PLANE] = (u16)pl[p][0];
-          r0[p * PLANE + PITCH] = (u16)(pl[p][0] >> 16);
-          r0[p * PLANE + 2 * PITCH] = (u16)pl[p][1];
-          r0[p * PLANE + 3 * PITCH] = (u16)(pl[p][1] >> 16);
-        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float2 p = cmul(dc[u][v], sv[u][v]);
@@ -576,17 +600,18 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
         if (u < nbo) {
           const int r = bi(u), c = bj(u);
           if (r == NC) {
-            srow[c] = svb[u];
-            if (c < NC) put_coef_srow<PITCH>(sK, c, svb[u]);
+            nxt.srow[c] = svb[u];
+            if (c < NC) put_coef_srow<PITCH>(nxt.sK, c, svb[u]);
           }
           if (c == NC) {
-            scol[r] = svb[u];
-            if (r < NC) put_coef_scol<PITCH>(sK, r, svb[u]);
+            nxt.scol[r] = svb[u];
+            if (r < NC) put_coef_scol<PITCH>(nxt.sK, r, svb[u]);
           }
         }
       }
       if (st < 3) {
         __syncthreads();   // S of the next stage is complete
+        ph ^= 1;
         continue;
       }
 
@@ -640,6 +665,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
         }
       }
       __syncthreads();   // S of the next step (and the record partials) are complete
+      ph ^= 1;
       if (rec_now && a.obs != nullptr && w == W - 1) {
         const int rec = g / a.nout - 1;
         int e0 = lane;
