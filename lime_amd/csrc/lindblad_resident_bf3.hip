@@ -483,8 +483,10 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       const float keep = (st == 0) ? 0.f : 1.f;   // the increment restarts at stage 0
       float2 sv[2][4], svb[2];
       if (st < 3) {
-        // next stage operand S = rho + c k (FP64, rounded to FP32, then split)
+        // next stage operand S = rho + c k, formed in FP32 (it is split into BF16 pairs, 2^-17
+        // relative, right after; the state itself stays FP64)
         const double cnext = (st == 2) ? dt : 0.5 * dt;
+        const float cnf = (float)cnext;
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -492,8 +494,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             const float kr = ip[u][v] - in[u][v], ki = -(rp[u][v] - rn[u][v]);
             inc[u][v].x = fmaf(cacc, kr, keep * inc[u][v].x);
             inc[u][v].y = fmaf(cacc, ki, keep * inc[u][v].y);
-            sv[u][v] = make_float2((float)(rho[u][v].x + cnext * (double)kr),
-                                   (float)(rho[u][v].y + cnext * (double)ki));
+            sv[u][v] = make_float2(fmaf(cnf, kr, (float)rho[u][v].x), fmaf(cnf, ki, (float)rho[u][v].y));
           }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
