@@ -141,14 +141,15 @@ struct Bf3Layout {
   static constexpr int W = 8;                        // waves, two core tiles each
   static constexpr int NC = 64;                      // core size = border index (n = 65)
   static constexpr int KP = 64;                      // MFMA reduction extent
-  static constexpr int PITCH = KP + 8;               // BF16 per plane row: 16-byte rows, and
-                                                     // 16 rows at one k hit distinct banks
-  static constexpr int PLANE = NC * PITCH;
+  // BF16 per plane row (multiples of 8: 16-byte rows for the fragment reads), chosen by
+  // measured LDS bank conflicts: H_eff planes, S column planes, coefficient rows
+  static constexpr int HP = KP + 16, SP = KP + 16, KPT = KP + 16;
+  static constexpr int HPLANE = NC * HP, SPLANE = NC * SP;
   static constexpr int NCP = NC + 2;                 // side-array length (keeps 16-byte alignment)
   static constexpr int NBO = 2 * NC + 2;             // border-output slots (2 NC + 1 used)
   // one stage buffer: S column planes, the coefficient operands (32 rows), S[NC][:], S[:][NC]
-  static constexpr size_t kStageBytes = (size_t)(4 * PLANE + 32 * PITCH) * 2 + (size_t)2 * NCP * 8;
-  static constexpr size_t kHBytes = (size_t)4 * PLANE * 2;   // H_eff row planes
+  static constexpr size_t kStageBytes = (size_t)(4 * SPLANE + 32 * KPT) * 2 + (size_t)2 * NCP * 8;
+  static constexpr size_t kHBytes = (size_t)4 * HPLANE * 2;   // H_eff row planes
   // H_eff planes, two stage buffers, H[:][NC] and H[NC][:], border-output state (FP64 rho,
   // FP32 increment, D)
   static constexpr size_t kFixedBytes = kHBytes + 2 * kStageBytes + (size_t)2 * NCP * 8 + (size_t)NBO * 32;
@@ -166,8 +167,8 @@ __device__ __forceinline__ Bf3Stage bf3_stage(unsigned char* base) {
   using L = Bf3Layout;
   Bf3Stage b;
   b.sC = reinterpret_cast<u16*>(base);
-  b.sK = b.sC + 4 * L::PLANE;
-  b.srow = reinterpret_cast<float2*>(b.sK + 32 * L::PITCH);
+  b.sK = b.sC + 4 * L::SPLANE;
+  b.srow = reinterpret_cast<float2*>(b.sK + 32 * L::KPT);
   b.scol = b.srow + L::NCP;
   return b;
 }
@@ -180,7 +181,7 @@ typedef __attribute__((address_space(3))) short4v lds_short4;
 // positions I0 + 4 (lr % 4) .. + 3
 __device__ __forceinline__ u16x8 frag_rows(const u16* plane, int off) {
   const short4v a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(plane + off));
-  const short4v b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(plane + off + 4 * Bf3Layout::PITCH));
+  const short4v b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(plane + off + 4 * Bf3Layout::SP));
   return u16x8{(u16)a[0], (u16)a[1], (u16)a[2], (u16)a[3], (u16)b[0], (u16)b[1], (u16)b[2], (u16)b[3]};
 }
 
@@ -190,7 +191,8 @@ template <bool REC>
 __global__ void __launch_bounds__(Bf3Layout::W * 64) __attribute__((amdgpu_waves_per_eu(2)))
 lindblad_rk4_resident_bf3(ResidentArgs a) {
   using L = Bf3Layout;
-  constexpr int W = L::W, NC = L::NC, KP = L::KP, PITCH = L::PITCH, PLANE = L::PLANE;
+  constexpr int W = L::W, NC = L::NC, KP = L::KP, HP = L::HP, SP = L::SP, KPT = L::KPT;
+  constexpr int HPLANE = L::HPLANE, SPLANE = L::SPLANE;
   constexpr int T = W * 64;
 
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
@@ -236,20 +238,20 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       const float2 h0 = H[i * n + k], h1 = H[i * n + k + 1];
       split2(h0.x, h1.x, xh, xl);
       split2(h0.y, h1.y, yh, yl);
-      u16* hr = sH + i * PITCH + k;
+      u16* hr = sH + i * HP + k;
       *reinterpret_cast<unsigned*>(hr) = xh;
-      *reinterpret_cast<unsigned*>(hr + PLANE) = xl;
-      *reinterpret_cast<unsigned*>(hr + 2 * PLANE) = yh;
-      *reinterpret_cast<unsigned*>(hr + 3 * PLANE) = yl;
+      *reinterpret_cast<unsigned*>(hr + HPLANE) = xl;
+      *reinterpret_cast<unsigned*>(hr + 2 * HPLANE) = yh;
+      *reinterpret_cast<unsigned*>(hr + 3 * HPLANE) = yl;
       // S[k][i], S[k + 1][i]: column plane row i, positions k, k + 1
       const double2 r0 = R[k * n + i], r1 = R[(k + 1) * n + i];
       split2((float)r0.x, (float)r1.x, xh, xl);
       split2((float)r0.y, (float)r1.y, yh, yl);
-      u16* cr = s0.sC + i * PITCH + k;
+      u16* cr = s0.sC + i * SP + k;
       *reinterpret_cast<unsigned*>(cr) = xh;
-      *reinterpret_cast<unsigned*>(cr + PLANE) = xl;
-      *reinterpret_cast<unsigned*>(cr + 2 * PLANE) = yh;
-      *reinterpret_cast<unsigned*>(cr + 3 * PLANE) = yl;
+      *reinterpret_cast<unsigned*>(cr + SPLANE) = xl;
+      *reinterpret_cast<unsigned*>(cr + 2 * SPLANE) = yh;
+      *reinterpret_cast<unsigned*>(cr + 3 * SPLANE) = yl;
     }
     for (int k = tid; k <= NC; k += T) {
       const float2 hc = H[k * n + NC], hr = H[NC * n + k];
@@ -260,10 +262,10 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       s0.srow[k] = sr;
       s0.scol[k] = sc;
       if (k < NC) {
-        put_coef_h<PITCH>(s0.sK, k, hr);   // the constant operands, in both buffers
-        put_coef_h<PITCH>(s1.sK, k, hr);
-        put_coef_srow<PITCH>(s0.sK, k, sr);
-        put_coef_scol<PITCH>(s0.sK, k, sc);
+        put_coef_h<KPT>(s0.sK, k, hr);   // the constant operands, in both buffers
+        put_coef_h<KPT>(s1.sK, k, hr);
+        put_coef_srow<KPT>(s0.sK, k, sr);
+        put_coef_scol<KPT>(s0.sK, k, sc);
       }
     }
   }
@@ -322,10 +324,11 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
   const double dt = a.dt;
   const bool renorm = (a.flags & 1) != 0;
   const bool record_any = REC && a.nout > 0 && (a.obs != nullptr || a.states != nullptr);
-  const int aoff = (I0 + lr) * PITCH + 8 * lg;                    // H rows of tile row ti (A)
-  const int boff = (J0 + lr) * PITCH + 8 * lg;                    // plane rows of the wave's first tile column (B)
-  const int troff = (8 * lg + (lr >> 2)) * PITCH + I0 + 4 * (lr & 3);   // S rows of tile row ti (A, transposed reads)
-  const int koff = (lr & 1) * PITCH + 8 * lg;    // coefficient rows (rows 2..15 repeat 0, 1:
+  const int aoff = (I0 + lr) * HP + 8 * lg;                    // H rows of tile row ti (A)
+  const int bHoff = (J0 + lr) * HP + 8 * lg;                   // H rows of the wave's first tile column (B)
+  const int bSoff = (J0 + lr) * SP + 8 * lg;                   // S columns of the wave's first tile column (B)
+  const int troff = (8 * lg + (lr >> 2)) * SP + I0 + 4 * (lr & 3);   // S rows of tile row ti (A, transposed reads)
+  const int koff = (lr & 1) * KPT + 8 * lg;    // coefficient rows (rows 2..15 repeat 0, 1:
                                                  // their outputs are not used)
   auto frag = [&](const u16* base, int off) { return *reinterpret_cast<const u16x8*>(base + off); };
 
@@ -360,16 +363,16 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       for (int k0 = 0; k0 < KP; k0 += 32) {
         // H S: A = H rows of tile row ti (shared), B = S columns of the wave's two tiles
         {
-          const u16x8 hxh = frag(sH, aoff + k0), hxl = frag(sH + PLANE, aoff + k0);
-          const u16x8 hyh = frag(sH + 2 * PLANE, aoff + k0), hyl = frag(sH + 3 * PLANE, aoff + k0);
+          const u16x8 hxh = frag(sH, aoff + k0), hxl = frag(sH + HPLANE, aoff + k0);
+          const u16x8 hyh = frag(sH + 2 * HPLANE, aoff + k0), hyl = frag(sH + 3 * HPLANE, aoff + k0);
           u16x8 sxh[2], sxl[2], syh[2], syl[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
-            const int o = boff + 16 * u * PITCH + k0;
+            const int o = bSoff + 16 * u * SP + k0;
             sxh[u] = frag(sC, o);
-            sxl[u] = frag(sC + PLANE, o);
-            syh[u] = frag(sC + 2 * PLANE, o);
-            syl[u] = frag(sC + 3 * PLANE, o);
+            sxl[u] = frag(sC + SPLANE, o);
+            syh[u] = frag(sC + 2 * SPLANE, o);
+            syl[u] = frag(sC + 3 * SPLANE, o);
           }
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -379,8 +382,8 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             ip[u] = mfma3(hyh, hyl, sxh[u], sxl[u], ip[u]);   // Hy Sx
           }
           if (rowsk) {
-            const u16x8 k0h = frag(sK, koff + k0), k0l = frag(sK, koff + 2 * PITCH + k0);
-            const u16x8 k1h = frag(sK, koff + 4 * PITCH + k0), k1l = frag(sK, koff + 6 * PITCH + k0);
+            const u16x8 k0h = frag(sK, koff + k0), k0l = frag(sK, koff + 2 * KPT + k0);
+            const u16x8 k1h = frag(sK, koff + 4 * KPT + k0), k1l = frag(sK, koff + 6 * KPT + k0);
             if (usk == 0) {
               q = mfma3(k0h, k0l, sxh[0], sxl[0], q);
               q = mfma3(k1h, k1l, syh[0], syl[0], q);
@@ -389,24 +392,24 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
               q = mfma3(k1h, k1l, syh[1], syl[1], q);
             }
           } else {
-            q = mfma3(hxh, hxl, frag(sK, koff + 16 * PITCH + k0), frag(sK, koff + 18 * PITCH + k0), q);
-            q = mfma3(hyh, hyl, frag(sK, koff + 20 * PITCH + k0), frag(sK, koff + 22 * PITCH + k0), q);
+            q = mfma3(hxh, hxl, frag(sK, koff + 16 * KPT + k0), frag(sK, koff + 18 * KPT + k0), q);
+            q = mfma3(hyh, hyl, frag(sK, koff + 20 * KPT + k0), frag(sK, koff + 22 * KPT + k0), q);
           }
         }
         // S H^dag: A = S rows of tile row ti (shared; transposed reads of the column planes),
         // B[k][j] = conj(H[j][k]) from H rows of the wave's tile columns
         {
-          const int to = troff + k0 * PITCH;
-          const u16x8 zxh = frag_rows(sC, to), zxl = frag_rows(sC + PLANE, to);
-          const u16x8 zyh = frag_rows(sC + 2 * PLANE, to), zyl = frag_rows(sC + 3 * PLANE, to);
+          const int to = troff + k0 * SP;
+          const u16x8 zxh = frag_rows(sC, to), zxl = frag_rows(sC + SPLANE, to);
+          const u16x8 zyh = frag_rows(sC + 2 * SPLANE, to), zyl = frag_rows(sC + 3 * SPLANE, to);
           u16x8 gxh[2], gxl[2], gyh[2], gyl[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
-            const int o = boff + 16 * u * PITCH + k0;
+            const int o = bHoff + 16 * u * HP + k0;
             gxh[u] = frag(sH, o);
-            gxl[u] = frag(sH + PLANE, o);
-            gyh[u] = frag(sH + 2 * PLANE, o);
-            gyl[u] = frag(sH + 3 * PLANE, o);
+            gxl[u] = frag(sH + HPLANE, o);
+            gyh[u] = frag(sH + 2 * HPLANE, o);
+            gyl[u] = frag(sH + 3 * HPLANE, o);
           }
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -416,8 +419,8 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             in[u] = mfma3(zyh, zyl, gxh[u], gxl[u], in[u]);   // Sy Hx'
           }
           if (rowsk) {
-            const u16x8 k2h = frag(sK, koff + 8 * PITCH + k0), k2l = frag(sK, koff + 10 * PITCH + k0);
-            const u16x8 k3h = frag(sK, koff + 12 * PITCH + k0), k3l = frag(sK, koff + 14 * PITCH + k0);
+            const u16x8 k2h = frag(sK, koff + 8 * KPT + k0), k2l = frag(sK, koff + 10 * KPT + k0);
+            const u16x8 k3h = frag(sK, koff + 12 * KPT + k0), k3l = frag(sK, koff + 14 * KPT + k0);
             if (usk == 0) {
               q = mfma3(k2h, k2l, gxh[0], gxl[0], q);
               q = mfma3(k3h, k3l, gyh[0], gyl[0], q);
@@ -426,8 +429,8 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
               q = mfma3(k3h, k3l, gyh[1], gyl[1], q);
             }
           } else {
-            q = mfma3(zxh, zxl, frag(sK, koff + 24 * PITCH + k0), frag(sK, koff + 26 * PITCH + k0), q);
-            q = mfma3(zyh, zyl, frag(sK, koff + 28 * PITCH + k0), frag(sK, koff + 30 * PITCH + k0), q);
+            q = mfma3(zxh, zxl, frag(sK, koff + 24 * KPT + k0), frag(sK, koff + 26 * KPT + k0), q);
+            q = mfma3(zyh, zyl, frag(sK, koff + 28 * KPT + k0), frag(sK, koff + 30 * KPT + k0), q);
           }
         }
       }
@@ -584,11 +587,11 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
         split2(sv[u][0].y, sv[u][1].y, yh0, yl0);
         split2(sv[u][2].x, sv[u][3].x, xh1, xl1);
         split2(sv[u][2].y, sv[u][3].y, yh1, yl1);
-        u16* c0 = nxt.sC + (J0 + 16 * u + lr) * PITCH + I0 + 4 * lg;   // column plane row j, positions i..i+3
+        u16* c0 = nxt.sC + (J0 + 16 * u + lr) * SP + I0 + 4 * lg;   // column plane row j, positions i..i+3
         *reinterpret_cast<uint2*>(c0) = make_uint2(xh0, xh1);
-        *reinterpret_cast<uint2*>(c0 + PLANE) = make_uint2(xl0, xl1);
-        *reinterpret_cast<uint2*>(c0 + 2 * PLANE) = make_uint2(yh0, yh1);
-        *reinterpret_cast<uint2*>(c0 + 3 * PLANE) = make_uint2(yl0, yl1);
+        *reinterpret_cast<uint2*>(c0 + SPLANE) = make_uint2(xl0, xl1);
+        *reinterpret_cast<uint2*>(c0 + 2 * SPLANE) = make_uint2(yh0, yh1);
+        *reinterpret_cast<uint2*>(c0 + 3 * SPLANE) = make_uint2(yl0, yl1);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const float2 p = cmul(dc[u][v], sv[u][v]);
@@ -602,11 +605,11 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
           const int r = bi(u), c = bj(u);
           if (r == NC) {
             nxt.srow[c] = svb[u];
-            if (c < NC) put_coef_srow<PITCH>(nxt.sK, c, svb[u]);
+            if (c < NC) put_coef_srow<KPT>(nxt.sK, c, svb[u]);
           }
           if (c == NC) {
             nxt.scol[r] = svb[u];
-            if (r < NC) put_coef_scol<PITCH>(nxt.sK, r, svb[u]);
+            if (r < NC) put_coef_scol<KPT>(nxt.sK, r, svb[u]);
           }
         }
       }
