@@ -298,8 +298,9 @@ int lime_adjoint_contract_pairs(int32_t batch, int32_t n, int32_t pairs, const v
   if (batch < 0 || n < 1) return fail(LIME_ERR_ARG, "lime_adjoint_contract: bad batch/n");
   if (pairs < 1 || pairs > lime::kMaxContractPairs)
     return fail(LIME_ERR_ARG, "lime_adjoint_contract_pairs: pairs must be in 1..4");
-  if (lime::contract_lds_bytes(n) > 160 * 1024)
-    return fail(LIME_ERR_UNSUPPORTED, "lime_adjoint_contract: n > 930 is not built");
+  if (lime::contract_lds_bytes(n) == 0)
+    return fail(LIME_ERR_UNSUPPORTED, "lime_adjoint_contract: n = " + std::to_string(n) +
+                                          " exceeds the contraction's LDS (n <= 1996)");
   if (batch == 0) return LIME_OK;
   if (!kappa || !s || !out_a || !out_b || (n > 1 && !out_u))
     return fail(LIME_ERR_ARG, "lime_adjoint_contract: null pointer");

@@ -845,93 +845,127 @@ lindblad_ragged_persistent(RaggedArgs r) {
 //   out_b[j] += w (P_jj - (sum_m P_jm + sum_m P_mj) / 2)    -> d/d(dephasing rate of j)
 //   out_u[j] += w (sum_m conj(k_jm) s_j+1,m + conj(k_j+1,m) s_jm
 //                  - sum_m conj(k_m,j+1) s_mj + conj(k_mj) s_m,j+1)   -> d/dH_j,j+1 (times -i)
-// One 1024-thread workgroup per operator, every global read coalesced: a column sweep
-// (thread per column, the rows split into cq = 1024 / n bands (at most 8) whose partial
-// sums meet in LDS in a fixed order) gives the column sums, the diagonal and the
-// column-pair terms; a row sweep (wave per row, lanes along the row, FP64 wave reductions)
-// the row sums and row-pair terms.  Deterministic: no atomics.
+// One 1024-thread workgroup per operator and ONE pass over kappa and s: wave w sweeps the
+// rows of its band (ceil(n / 16) consecutive rows), 64 columns at a time, lanes along the
+// row (coalesced).  Row sums and row-pair terms are FP64 wave sums, added by the row's own
+// wave into LDS; the previous row of the row-pair terms stays in registers, the next column
+// of the column-pair terms is the neighbouring lane's cache line.  Column sums and column-pair terms stay in the lane's registers
+// for the band and meet in LDS in a fixed order (waves in NS slots, slot-sharing waves in
+// rounds).  Deterministic: no atomics.
 constexpr int CT_THREADS = 1024;
-constexpr int CQ_MAX = 8;
-__host__ __device__ inline int contract_bands(int n) {
-  const int q = CT_THREADS / (n > 0 ? n : 1);
-  return q < 1 ? 1 : (q > CQ_MAX ? CQ_MAX : q);
+constexpr int CT_WAVES = CT_THREADS / 64;
+constexpr size_t CT_LDS = 156 * 1024;
+// column-partial slots that fit the LDS next to the row arrays
+__host__ __device__ inline int contract_slots(int n) {
+  const long long free_bytes = (long long)CT_LDS - 3ll * n * 16;
+  long long s = free_bytes / (2ll * n * 16);
+  if (s > CT_WAVES) s = CT_WAVES;
+  return s < 1 ? 0 : (int)s;
 }
 
 __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
   extern __shared__ __attribute__((aligned(16))) unsigned char craw[];
   const int b = blockIdx.x, n = c.n;
   const size_t nn = (size_t)n * n;
-  const int CQ = contract_bands(n);
-  const int TQ = CT_THREADS / CQ;                    // threads per band (>= n when n <= 1024)
-  double2* cs = reinterpret_cast<double2*>(craw);   // [CQ][n] column-sum partials
-  double2* cp = cs + CQ * n;                         // [CQ][n] column-pair partials
-  double2* dg = cp + CQ * n;                         // diagonal
-  double2* rs = dg + n;                              // row sums
-  double2* rp = rs + n;                              // row-pair terms
-  const int q = threadIdx.x / TQ, tq = threadIdx.x % TQ;
-  const int rows = (n + CQ - 1) / CQ, m0 = q * rows, m1 = q < CQ ? min(n, m0 + rows) : m0;
-  // every pair (kappa_p, s_p) adds into the same partial sums, pair by pair in a fixed order
-  for (int j = tq; j < n && q < CQ; j += TQ) {
-    double sr = 0, si = 0, pr = 0, pi = 0, dr = 0, di = 0;
-    for (int pp = 0; pp < c.pairs; ++pp) {
-      const double2* K = c.kappa[pp] + (size_t)b * nn;
-      const double2* S = c.s[pp] + (size_t)b * nn;
-      for (int m = m0; m < m1; ++m) {
-        const double2 k = K[(size_t)m * n + j], x = S[(size_t)m * n + j];
-        sr += k.x * x.x + k.y * x.y;
-        si += k.x * x.y - k.y * x.x;
-        if (j + 1 < n) {
-          const double2 k1 = K[(size_t)m * n + j + 1], x1 = S[(size_t)m * n + j + 1];
-          pr += (k1.x * x.x + k1.y * x.y) + (k.x * x1.x + k.y * x1.y);
-          pi += (k1.x * x.y - k1.y * x.x) + (k.x * x1.y - k.y * x1.x);
-        }
-      }
-      if (q == 0) {
-        const double2 kd = K[(size_t)j * n + j], sd = S[(size_t)j * n + j];
-        dr += kd.x * sd.x + kd.y * sd.y;
-        di += kd.x * sd.y - kd.y * sd.x;
+  const int NS = contract_slots(n);
+  double2* colC = reinterpret_cast<double2*>(craw);   // [NS][n] column-sum partials
+  double2* colP = colC + (size_t)NS * n;               // [NS][n] column-pair partials
+  double2* rowR = colP + (size_t)NS * n;               // [n] row sums
+  double2* rowP = rowR + n;                            // [n] row-pair terms
+  double2* dia = rowP + n;                             // [n] diagonal
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int band = (n + CT_WAVES - 1) / CT_WAVES;
+  const int r0 = min(n, wave * band), r1 = min(n, r0 + band);
+  for (int i = threadIdx.x; i < 3 * n; i += CT_THREADS) rowR[i] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int rounds = (CT_WAVES + NS - 1) / NS;
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    const int m = c0 + lane;
+    const bool in = m < n, has_next = m + 1 < n;
+    double cr = 0, ci = 0, qr = 0, qi = 0;      // column sum and column-pair term of column m
+    // kappa and s of the previous row at column m, per pair (registers: each element is read
+    // from memory once, plus one row above the band)
+    double2 kp[kMaxContractPairs], xp[kMaxContractPairs];
+#pragma unroll
+    for (int pp = 0; pp < kMaxContractPairs; ++pp) {
+      kp[pp] = make_double2(0.0, 0.0);
+      xp[pp] = make_double2(0.0, 0.0);
+      if (pp < c.pairs && in && r0 > 0 && r0 < r1) {
+        kp[pp] = c.kappa[pp][(size_t)b * nn + (size_t)(r0 - 1) * n + m];
+        xp[pp] = c.s[pp][(size_t)b * nn + (size_t)(r0 - 1) * n + m];
       }
     }
-    cs[q * n + j] = make_double2(sr, si);
-    cp[q * n + j] = make_double2(pr, pi);
-    if (q == 0) dg[j] = make_double2(dr, di);
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int r = wave; r < n; r += nw) {
-    double sr = 0, si = 0, pr = 0, pi = 0;
-    for (int pp = 0; pp < c.pairs; ++pp) {
-      const double2* K = c.kappa[pp] + (size_t)b * nn;
-      const double2* S = c.s[pp] + (size_t)b * nn;
-      for (int m = lane; m < n; m += 64) {
-        const double2 k = K[(size_t)r * n + m], x = S[(size_t)r * n + m];
-        sr += k.x * x.x + k.y * x.y;
-        si += k.x * x.y - k.y * x.x;
-        if (r + 1 < n) {
-          const double2 k1 = K[(size_t)(r + 1) * n + m], x1 = S[(size_t)(r + 1) * n + m];
-          pr += (k.x * x1.x + k.y * x1.y) + (k1.x * x.x + k1.y * x.y);
-          pi += (k.x * x1.y - k.y * x1.x) + (k1.x * x.y - k1.y * x.x);
+    for (int r = r0; r < r1; ++r) {
+      double pr = 0, pi = 0, tr = 0, ti = 0;    // P_rm; the pair term of rows r - 1, r at m
+      double ur = 0, ui = 0;                     // the pair term of columns m, m + 1 at row r
+#pragma unroll
+      for (int pp = 0; pp < kMaxContractPairs; ++pp) {
+        if (pp < c.pairs && in) {
+          const double2* K = c.kappa[pp] + (size_t)b * nn;
+          const double2* S = c.s[pp] + (size_t)b * nn;
+          const double2 k = K[(size_t)r * n + m], x = S[(size_t)r * n + m];
+          pr += k.x * x.x + k.y * x.y;
+          pi += k.x * x.y - k.y * x.x;
+          if (has_next) {   // the next column of the same row: the neighbouring lane's line
+            const double2 k1 = K[(size_t)r * n + m + 1], x1 = S[(size_t)r * n + m + 1];
+            ur += (k1.x * x.x + k1.y * x.y) + (k.x * x1.x + k.y * x1.y);
+            ui += (k1.x * x.y - k1.y * x.x) + (k.x * x1.y - k.y * x1.x);
+          }
+          if (r > 0) {
+            tr += (kp[pp].x * x.x + kp[pp].y * x.y) + (k.x * xp[pp].x + k.y * xp[pp].y);
+            ti += (kp[pp].x * x.y - kp[pp].y * x.x) + (k.x * xp[pp].y - k.y * xp[pp].x);
+          }
+          kp[pp] = k;
+          xp[pp] = x;
+        }
+      }
+      cr += pr;
+      ci += pi;
+      qr += ur;
+      qi += ui;
+      if (in && m == r) dia[r] = make_double2(pr, pi);
+      // row quantities: this wave owns row r (and the pair term of rows r - 1, r)
+      const double sr = wave_sum(pr), si = wave_sum(pi);
+      const double wr = wave_sum(tr), wi = wave_sum(ti);
+      if (lane == 0) {
+        rowR[r].x += sr;
+        rowR[r].y += si;
+        if (r > 0) {
+          rowP[r - 1].x += wr;
+          rowP[r - 1].y += wi;
         }
       }
     }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    pr = wave_sum(pr);
-    pi = wave_sum(pi);
-    if (lane == 0) {
-      rs[r] = make_double2(sr, si);
-      rp[r] = make_double2(pr, pi);
+    // column partials of this 64-column block: slot wave % NS, waves sharing a slot in
+    // rounds (round 0 stores, later rounds add), so the sum order is fixed
+    for (int k = 0; k < rounds; ++k) {
+      if (wave / NS == k && in) {
+        double2* C = colC + (size_t)(wave % NS) * n + m;
+        double2* Q = colP + (size_t)(wave % NS) * n + m;
+        if (k == 0) {
+          *C = make_double2(cr, ci);
+          *Q = make_double2(qr, qi);
+        } else {
+          C->x += cr;
+          C->y += ci;
+          Q->x += qr;
+          Q->y += qi;
+        }
+      }
+      if (rounds > 1) __syncthreads();
     }
   }
   __syncthreads();
+  const int used = min(NS, CT_WAVES);
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     double2 Cc = make_double2(0.0, 0.0), Cp = make_double2(0.0, 0.0);
-    for (int x = 0; x < CQ; ++x) {
-      Cc.x += cs[x * n + j].x;
-      Cc.y += cs[x * n + j].y;
-      Cp.x += cp[x * n + j].x;
-      Cp.y += cp[x * n + j].y;
+    for (int x = 0; x < used; ++x) {
+      Cc.x += colC[(size_t)x * n + j].x;
+      Cc.y += colC[(size_t)x * n + j].y;
+      Cp.x += colP[(size_t)x * n + j].x;
+      Cp.y += colP[(size_t)x * n + j].y;
     }
-    const double2 R = rs[j], D = dg[j];
+    const double2 R = rowR[j], D = dia[j];
     double2* A = c.out_a + (size_t)b * n + j;
     A->x += c.weight * (R.x - Cc.x);
     A->y += c.weight * (R.y - Cc.y);
@@ -940,8 +974,8 @@ __global__ void __launch_bounds__(CT_THREADS) adjoint_contract(ContractArgs c) {
     B->y += c.weight * (D.y - 0.5 * (R.y + Cc.y));
     if (j + 1 < n) {
       double2* U = c.out_u + (size_t)b * (n - 1) + j;
-      U->x += c.weight * (rp[j].x - Cp.x);
-      U->y += c.weight * (rp[j].y - Cp.y);
+      U->x += c.weight * (rowP[j].x - Cp.x);
+      U->y += c.weight * (rowP[j].y - Cp.y);
     }
   }
 }
@@ -979,7 +1013,11 @@ hipError_t seed_launch(const SeedArgs& s, hipStream_t stream) {
   return hipGetLastError();
 }
 
-size_t contract_lds_bytes(int n) { return (size_t)(2 * contract_bands(n) + 3) * n * sizeof(double2); }
+// 0 when no slot fits (n too large for the contraction's LDS)
+size_t contract_lds_bytes(int n) {
+  const int ns = contract_slots(n);
+  return ns < 1 ? 0 : (size_t)(2 * ns + 3) * n * sizeof(double2);
+}
 
 hipError_t contract_launch(const ContractArgs& c, hipStream_t stream) {
   const size_t lds = contract_lds_bytes(c.n);

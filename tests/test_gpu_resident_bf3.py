@@ -122,3 +122,19 @@ def test_chain_populations_long_run():
     err = np.max(np.abs(obs[0].cpu().numpy() - ref))
     print(f"resident 3xbf16 chain populations over {nt} steps: max error {err:.3e}")
     assert err < 1e-6
+
+
+def test_chunked_and_strided_records_equal_single_launch():
+    """Records every 3rd step and launches of 7 steps: the same bits as one launch (the step
+    offset and the record index are carried across launches)."""
+    rng = np.random.default_rng(91)
+    H = rand_herm(rng, N)
+    c_ops = [np.sqrt(0.03) * np.diag(np.eye(N)[i]) for i in range(N)]
+    X0 = np.stack([rand_dm(rng, N), rand_c(rng, N, N) / N, rand_dm(rng, N)])
+    A = [rand_herm(rng, N), rand_c(rng, N, N)]
+    prop = _prop()(H, c_ops, group=3, precision="3xbf16")
+    r1, o1, s1 = prop.run(X0, 0.02, 30, e_ops=A, nout=3, store_states=True)
+    r2, o2, s2 = prop.run(X0, 0.02, 30, e_ops=A, nout=3, store_states=True, chunk=7)
+    assert torch.equal(r1, r2) and torch.equal(o1, o2) and torch.equal(s1, s2)
+    ref, ref_obs, _ = ol.propagate(H, X0[1], c_ops, 0.02, 30, A, nout=3)
+    assert np.max(np.abs(o1[1].cpu().numpy() - ref_obs)) < 1e-5 * np.max(np.abs(ref_obs))
