@@ -138,3 +138,26 @@ def test_chunked_and_strided_records_equal_single_launch():
     assert torch.equal(r1, r2) and torch.equal(o1, o2) and torch.equal(s1, s2)
     ref, ref_obs, _ = ol.propagate(H, X0[1], c_ops, 0.02, 30, A, nout=3)
     assert np.max(np.abs(o1[1].cpu().numpy() - ref_obs)) < 1e-5 * np.max(np.abs(ref_obs))
+
+
+def test_renormalisation_records_after_scaling():
+    """Trace renormalisation with records: Re Tr rho = 1 after every step, the recorded
+    states and observables are those of the renormalised state (the border corner included),
+    and the physics agrees with the trace-conserving FP64 oracle."""
+    rng = np.random.default_rng(123)
+    H = rand_herm(rng, N)
+    c_ops = [np.sqrt(0.05) * np.diag(np.eye(N)[i]) for i in range(N)]
+    rho0 = rand_dm(rng, N)
+    E = [np.diag(np.eye(N)[i]).astype(complex) for i in (0, N // 2, N - 1)]
+    dt, nt = 0.05, 40
+    rho, obs, states = _prop()(H, c_ops, precision="3xbf16").run(rho0, dt, nt, e_ops=E, nout=8,
+                                                                  store_states=True, renorm=True)
+    r, st, ob = rho[0].cpu().numpy(), states[0].cpu().numpy(), obs[0].cpu().numpy()
+    assert abs(np.trace(r).real - 1) < 1e-13
+    for k in range(st.shape[0]):
+        assert abs(np.trace(st[k]).real - 1) < 1e-13
+        for e in range(len(E)):
+            assert abs(ob[k, e] - np.trace(E[e] @ st[k])) < 1e-12
+    assert np.allclose(st[-1], r, rtol=0, atol=0)
+    ref, _, _ = ol.propagate(H, rho0, c_ops, dt, nt)
+    assert np.max(np.abs(r - ref)) / np.max(np.abs(ref)) < 2e-6
