@@ -30,7 +30,6 @@ namespace lime {
 
 namespace {
 
-constexpr int KP = 16;          // K-panel width (complex elements) of the 3xBF16 staging
 constexpr int KW = 16;          // K-panel width of the FP32 / FP64 staging
 constexpr int KWP = KW + 1;         // LDS pitch of [row][k] panels (odd: conflict-free fragments)
 
@@ -190,28 +189,39 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
 }
 
 // ---- 3xBF16 split products
-// A panel row (and B panel column) of KP complex elements is stored as 96 BF16 "slots"
-// along the MFMA reduction axis: slots 4k..4k+3 carry the main terms of element k, slots
-// 64+2k, 64+2k+1 the hi*lo terms.  With a = ar + i ai, b = br + i bi and x = x_hi + x_lo:
-//   A  slots : (ar_hi, ar_lo, ai_hi, ai_lo)       | (ar_hi, ai_hi)
-//   Br slots : (br_hi, br_hi, -bi_hi, -bi_hi)     | (br_lo, -bi_lo)     -> sum = Re(a b) - lo*lo
-//   Bi slots : (bi_hi, bi_hi,  br_hi,  br_hi)     | (bi_lo,  br_lo)     -> sum = Im(a b) - lo*lo
-// so one 16x16 block needs 3 MFMAs (32 slots each) for the real and 3 for the imaginary
-// part per panel.  A lane of group g feeds slots 32m + 8g .. +7 of MFMA m for both operands,
-// which pairs A and B slot by slot whatever order the instruction sums them in.
-constexpr int BP = 104;         // LDS pitch (BF16 slots) of a split row/column: 96 + 8, 16-byte aligned
+// Every real operand is split x = hi + lo (v_cvt_pk_bf16_f32, round to nearest even; x - hi
+// is exact in FP32) as it is staged into LDS, each part in its own plane (Re hi, Re lo,
+// Im hi, Im lo), K-panels of KB = 32 complex elements (one 16x16x32 MFMA k-step).  A rows,
+// and the B operand of the conjugate-transpose modes (rows j of M), are planes [row][k]:
+// one 16-byte read per fragment.  The PLAIN B operand (rows k of M) is stored as planes
+// [k][column] and its fragments (8 consecutive k of one column) come through
+// ds_read_b64_tr_b16, whose result lane i, element e is element i % 4 of what lane
+// 4 e + i / 4 of the 16-lane group loaded.  Each real product X Y is Xlo Yhi + Xhi Ylo +
+// Xhi Yhi in FP32 (the lo*lo term, <= 2^-16 |X Y|, is dropped); 4 real products per complex
+// product into Re+ (cr), Re- (dr) and Im (ci) accumulators, with the conjugate-transpose
+// mode's sign staged into its B planes:
+//   PLAIN  (C += a b)          B = (bx, by):   Re += ax bx - ay by,  Im += ax by + ay bx
+//   NEG_AH (C -= a conj(h))    B = (-hx, hy):  Re += ax (-hx) - ay hy,  Im += ax hy + ay (-hx)
+//   I_AH   (C += i a conj(h))  B = (hx, hy):   Re += ax hy - ay hx,  Im += ax hx + ay hy
+constexpr int KB = 32;          // K-panel width (complex elements) of the 3xBF16 staging
+constexpr int PP = KB + 8;      // BF16 pitch of [row][k] planes (80-byte rows: 16 rows of a
+                                // fragment read hit distinct banks)
 
+typedef unsigned short u16;
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float float2v __attribute__((ext_vector_type(2)));
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) short4v lds_short4;
 
-// Split both parts of v into BF16 pairs x = hi + lo (each rounded to nearest even by
-// v_cvt_pk_bf16_f32; x - hi is exact in FP32).  Packed: h = re_hi | im_hi << 16,
-// l = re_lo | im_lo << 16.
-__device__ __forceinline__ void split_bf16(float2 v, unsigned& h, unsigned& l) {
-  const float2v x = {v.x, v.y};
+// BF16 pitch of the PLAIN B planes [k][column]
+template <int TILE>
+constexpr int pb_pitch() { return TILE + 8; }
+
+// h = hi(a) | hi(b) << 16, l = lo(a) | lo(b) << 16 with x = hi + lo
+__device__ __forceinline__ void split2(float a, float b, unsigned& h, unsigned& l) {
+  const float2v x = {a, b};
   const bf16x2 hb = __builtin_convertvector(x, bf16x2);
   const float2v hf = __builtin_convertvector(hb, float2v);
   const bf16x2 lb = __builtin_convertvector(x - hf, bf16x2);
@@ -219,98 +229,171 @@ __device__ __forceinline__ void split_bf16(float2 v, unsigned& h, unsigned& l) {
   l = __builtin_bit_cast(unsigned, lb);
 }
 
-__device__ __forceinline__ void put_a(unsigned short* row, int k, float2 v) {
-  unsigned h, l;
-  split_bf16(v, h, l);
-  // (ar_hi, ar_lo, ai_hi, ai_lo) | (ar_hi, ai_hi)
-  *reinterpret_cast<uint2*>(row + 4 * k) = make_uint2((h & 0xffffu) | (l << 16), (h >> 16) | (l & 0xffff0000u));
-  *reinterpret_cast<unsigned*>(row + 64 + 2 * k) = h;
+__device__ __forceinline__ floatx4 mfma_bf(u16x8 a, u16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
 }
 
-__device__ __forceinline__ void put_b(unsigned short* colr, unsigned short* coli, int k, float2 v) {
-  unsigned h, l;
-  split_bf16(v, h, l);
-  const unsigned rh = h & 0xffffu, ih = h >> 16, nih = ih ^ 0x8000u;
-  // Br: (br_hi, br_hi, -bi_hi, -bi_hi) | (br_lo, -bi_lo);  Bi: (bi_hi, bi_hi, br_hi, br_hi) | (bi_lo, br_lo)
-  *reinterpret_cast<uint2*>(colr + 4 * k) = make_uint2(rh | (rh << 16), nih | (nih << 16));
-  *reinterpret_cast<unsigned*>(colr + 64 + 2 * k) = l ^ 0x80000000u;
-  *reinterpret_cast<uint2*>(coli + 4 * k) = make_uint2(ih | (ih << 16), rh | (rh << 16));
-  *reinterpret_cast<unsigned*>(coli + 64 + 2 * k) = (l >> 16) | (l << 16);
+// c += X Y for split operands (small terms first)
+__device__ __forceinline__ floatx4 mfma3(u16x8 xh, u16x8 xl, u16x8 yh, u16x8 yl, floatx4 c) {
+  c = mfma_bf(xl, yh, c);
+  c = mfma_bf(xh, yl, c);
+  return mfma_bf(xh, yh, c);
 }
 
-// B element of a product term from the source operand: PLAIN b = M[k][j];
-// NEG_AH b = -conj(M[j][k]); I_AH b = i conj(M[j][k]).
-template <int MODE>
-__device__ __forceinline__ float2 b_elem(float2 m) {
-  if (MODE == PLAIN) return m;
-  if (MODE == NEG_AH) return make_float2(-m.x, m.y);
-  return make_float2(m.y, m.x);
+// Stage element pair (e0, e1) (consecutive along the plane row) at plane offset o; NEG
+// negates the real parts.
+template <int PLANE, bool NEG>
+__device__ __forceinline__ void put_pair(u16* sM, int o, float2v e0, float2v e1) {
+  unsigned xh, xl, yh, yl;
+  split2(e0.x, e1.x, xh, xl);
+  split2(e0.y, e1.y, yh, yl);
+  if (NEG) {
+    xh ^= 0x80008000u;
+    xl ^= 0x80008000u;
+  }
+  *reinterpret_cast<unsigned*>(sM + o) = xh;
+  *reinterpret_cast<unsigned*>(sM + PLANE + o) = xl;
+  *reinterpret_cast<unsigned*>(sM + 2 * PLANE + o) = yh;
+  *reinterpret_cast<unsigned*>(sM + 3 * PLANE + o) = yl;
+}
+
+// Element pairs of a ROWS x COLS panel of a row-major n x n matrix (rows r0.., cols c0..),
+// NP pairs per thread, pair q = tid + 256 i along the columns.
+template <int ROWS, int COLS>
+struct PanelPairs {
+  static constexpr int NP = ROWS * COLS / 2 / 256;
+};
+template <int ROWS, int COLS>
+__device__ __forceinline__ void load_pairs(const float2* __restrict__ M, int n, int r0, int c0,
+                                           float2v (&reg)[2 * PanelPairs<ROWS, COLS>::NP], int tid) {
+  const float2v* __restrict__ Mv = reinterpret_cast<const float2v*>(M);
+#pragma unroll
+  for (int i = 0; i < PanelPairs<ROWS, COLS>::NP; ++i) {
+    const int q = tid + 256 * i;
+    const int r = q / (COLS / 2), c = 2 * (q % (COLS / 2));
+    const int gr = r0 + r, gc = c0 + c;
+    float2v v0 = {0.f, 0.f}, v1 = {0.f, 0.f};
+    if (gr < n) {
+      if (gc < n) v0 = Mv[(size_t)gr * n + gc];
+      if (gc + 1 < n) v1 = Mv[(size_t)gr * n + gc + 1];
+    }
+    reg[2 * i] = v0;
+    reg[2 * i + 1] = v1;
+  }
+}
+template <int ROWS, int COLS, int PITCH, bool NEG>
+__device__ __forceinline__ void store_pairs(u16* sM, const float2v (&reg)[2 * PanelPairs<ROWS, COLS>::NP], int tid) {
+#pragma unroll
+  for (int i = 0; i < PanelPairs<ROWS, COLS>::NP; ++i) {
+    const int q = tid + 256 * i;
+    const int r = q / (COLS / 2), c = 2 * (q % (COLS / 2));
+    put_pair<ROWS * PITCH, NEG>(sM, r * PITCH + c, reg[2 * i], reg[2 * i + 1]);
+  }
 }
 
 template <int TILE, int MODE>
 __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, const float2* __restrict__ Bsrc, int n,
                                               int row0, int col0, unsigned char* smem,
                                               floatx4 (&cr)[TILE / 32][TILE / 32],
-                                              floatx4 (&ci)[TILE / 32][TILE / 32], int tid, int wr, int wc,
+                                              floatx4 (&ci)[TILE / 32][TILE / 32],
+                                              floatx4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
                                               int lr, int lg) {
   constexpr int BW = TILE / 32;
-  constexpr int NR = TILE * KP / 256;
+  constexpr int PB = pb_pitch<TILE>();
+  constexpr int APLANE = TILE * PP;
+  constexpr int NA = 2 * PanelPairs<TILE, KB>::NP;                       // A: TILE rows x KB
+  constexpr int NB = 2 * (MODE == PLAIN ? PanelPairs<KB, TILE>::NP : PanelPairs<TILE, KB>::NP);
   bool rlive[BW], clive[BW];
 #pragma unroll
   for (int p = 0; p < BW; ++p) {
     rlive[p] = row0 + wr * (TILE / 2) + p * 16 < n;
     clive[p] = col0 + wc * (TILE / 2) + p * 16 < n;
   }
-  unsigned short* sA = reinterpret_cast<unsigned short*>(smem);
-  unsigned short* sBr = sA + TILE * BP;
-  unsigned short* sBi = sBr + TILE * BP;
-  float2 ra[NR], rb[NR];
-  load_rows<TILE, KP>(A, n, row0, 0, ra, tid);
-  if (MODE == PLAIN) load_cols<TILE, KP>(Bsrc, n, 0, col0, rb, tid);
-  else load_rows<TILE, KP>(Bsrc, n, col0, 0, rb, tid);
-  for (int k0 = 0; k0 < n; k0 += KP) {
+  u16* sA = reinterpret_cast<u16*>(smem);
+  u16* sB = sA + 4 * APLANE;
+  constexpr int BPLANE = MODE == PLAIN ? KB * PB : TILE * PP;
+  // PF: the next panel is loaded into registers while the current one feeds the MFMAs.
+  // Measured at n = 512: +11 % for 32 x 32 tiles; for 64 x 64 tiles the 32 prefetch VGPRs
+  // cost a wave per SIMD (3 -> 2) and loading each panel when it is staged is 8 % faster.
+  constexpr bool PF = TILE == 32;
+  float2v ra[NA], rb[NB];
+  auto load_panel = [&](int k) {
+    load_pairs<TILE, KB>(A, n, row0, k, ra, tid);
+    if constexpr (MODE == PLAIN) load_pairs<KB, TILE>(Bsrc, n, k, col0, rb, tid);
+    else load_pairs<TILE, KB>(Bsrc, n, col0, k, rb, tid);
+  };
+  if constexpr (PF) load_panel(0);
+  for (int k0 = 0; k0 < n; k0 += KB) {
+    if constexpr (!PF) load_panel(k0);
     __syncthreads();                          // previous panel fully consumed
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      const int e = tid + q * 256;
-      put_a(sA + (e / KP) * BP, e % KP, ra[q]);
-      if (MODE == PLAIN) put_b(sBr + (e % TILE) * BP, sBi + (e % TILE) * BP, e / TILE, rb[q]);
-      else put_b(sBr + (e / KP) * BP, sBi + (e / KP) * BP, e % KP, b_elem<MODE>(rb[q]));
-    }
+    store_pairs<TILE, KB, PP, false>(sA, ra, tid);
+    if constexpr (MODE == PLAIN) store_pairs<KB, TILE, PB, false>(sB, rb, tid);
+    else store_pairs<TILE, KB, PP, MODE == NEG_AH>(sB, rb, tid);
     __syncthreads();
-    if (k0 + KP < n) {                        // prefetch the next panel into registers
-      load_rows<TILE, KP>(A, n, row0, k0 + KP, ra, tid);
-      if (MODE == PLAIN) load_cols<TILE, KP>(Bsrc, n, k0 + KP, col0, rb, tid);
-      else load_rows<TILE, KP>(Bsrc, n, col0, k0 + KP, rb, tid);
+    if constexpr (PF) {
+      if (k0 + KB < n) load_panel(k0 + KB);   // prefetch the next panel into registers
+    }
+    u16x8 a0h[BW], a0l[BW], a1h[BW], a1l[BW], b0h[BW], b0l[BW], b1h[BW], b1l[BW];
+#pragma unroll
+    for (int p = 0; p < BW; ++p) {
+      const int o = (wr * (TILE / 2) + p * 16 + lr) * PP + 8 * lg;
+      a0h[p] = *reinterpret_cast<const u16x8*>(sA + o);
+      a0l[p] = *reinterpret_cast<const u16x8*>(sA + APLANE + o);
+      a1h[p] = *reinterpret_cast<const u16x8*>(sA + 2 * APLANE + o);
+      a1l[p] = *reinterpret_cast<const u16x8*>(sA + 3 * APLANE + o);
     }
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      if (m == 1 && k0 + 8 >= n) continue;    // main slots of k0+8..k0+15: zeros past n
-      u16x8 x[BW], yr[BW], yi[BW];
+    for (int q = 0; q < BW; ++q) {
+      if constexpr (MODE == PLAIN) {
+        // transposed reads: lane (lg, lr) loads plane row 8 lg + lr / 4 (+ 4), columns
+        // c + 4 (lr % 4) .. + 3 of the block's 16 columns
+        const int o = (8 * lg + (lr >> 2)) * PB + wc * (TILE / 2) + q * 16 + 4 * (lr & 3);
+        u16x8 f[4];
 #pragma unroll
-      for (int p = 0; p < BW; ++p) {
-        x[p] = *reinterpret_cast<const u16x8*>(sA + (wr * (TILE / 2) + p * 16 + lr) * BP + 32 * m + 8 * lg);
-        const int c = (wc * (TILE / 2) + p * 16 + lr) * BP + 32 * m + 8 * lg;
-        yr[p] = *reinterpret_cast<const u16x8*>(sBr + c);
-        yi[p] = *reinterpret_cast<const u16x8*>(sBi + c);
-      }
-#pragma unroll
-      for (int p = 0; p < BW; ++p)
-#pragma unroll
-        for (int q = 0; q < BW; ++q) {
-          if (!(rlive[p] && clive[q])) continue;
-          const bf16x8 a = __builtin_bit_cast(bf16x8, x[p]);
-          cr[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, yr[q]), cr[p][q], 0, 0, 0);
-          ci[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, yi[q]), ci[p][q], 0, 0, 0);
+        for (int pl = 0; pl < 4; ++pl) {
+          const u16* base = sB + pl * BPLANE + o;
+          const short4v x = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(base));
+          const short4v y = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_short4*)(base + 4 * PB));
+          f[pl] = u16x8{(u16)x[0], (u16)x[1], (u16)x[2], (u16)x[3], (u16)y[0], (u16)y[1], (u16)y[2], (u16)y[3]};
         }
+        b0h[q] = f[0];
+        b0l[q] = f[1];
+        b1h[q] = f[2];
+        b1l[q] = f[3];
+      } else {
+        const int o = (wc * (TILE / 2) + q * 16 + lr) * PP + 8 * lg;
+        b0h[q] = *reinterpret_cast<const u16x8*>(sB + o);
+        b0l[q] = *reinterpret_cast<const u16x8*>(sB + BPLANE + o);
+        b1h[q] = *reinterpret_cast<const u16x8*>(sB + 2 * BPLANE + o);
+        b1l[q] = *reinterpret_cast<const u16x8*>(sB + 3 * BPLANE + o);
+      }
     }
+#pragma unroll
+    for (int p = 0; p < BW; ++p)
+#pragma unroll
+      for (int q = 0; q < BW; ++q) {
+        if (!(rlive[p] && clive[q])) continue;
+        if constexpr (MODE == I_AH) {
+          cr[p][q] = mfma3(a0h[p], a0l[p], b1h[q], b1l[q], cr[p][q]);
+          dr[p][q] = mfma3(a1h[p], a1l[p], b0h[q], b0l[q], dr[p][q]);
+          ci[p][q] = mfma3(a0h[p], a0l[p], b0h[q], b0l[q], ci[p][q]);
+          ci[p][q] = mfma3(a1h[p], a1l[p], b1h[q], b1l[q], ci[p][q]);
+        } else {
+          cr[p][q] = mfma3(a0h[p], a0l[p], b0h[q], b0l[q], cr[p][q]);
+          dr[p][q] = mfma3(a1h[p], a1l[p], b1h[q], b1l[q], dr[p][q]);
+          ci[p][q] = mfma3(a0h[p], a0l[p], b1h[q], b1l[q], ci[p][q]);
+          ci[p][q] = mfma3(a1h[p], a1l[p], b0h[q], b0l[q], ci[p][q]);
+        }
+      }
   }
 }
 
 // LDS bytes of a GEMM-staging area.
 template <int TILE, typename R, int P>
 constexpr int smem_bytes() {
-  return P == 1 ? 3 * TILE * BP * 2 : (TILE * KWP + smem_b<TILE>()) * (int)sizeof(typename CT<R>::C);
+  return P == 1 ? (4 * TILE * PP + 4 * (KB * pb_pitch<TILE>() > TILE * PP ? KB * pb_pitch<TILE>() : TILE * PP)) * 2
+                : (TILE * KWP + smem_b<TILE>()) * (int)sizeof(typename CT<R>::C);
 }
 
 // One product term in the selected arithmetic.
@@ -322,7 +405,7 @@ __device__ __forceinline__ void term(const typename CT<R>::C* __restrict__ A, co
                                      typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
                                      int lr, int lg) {
   if constexpr (P == 1) {
-    gemm_term_bf3<TILE, MODE>(A, B, n, row0, col0, smem, cr, ci, tid, wr, wc, lr, lg);
+    gemm_term_bf3<TILE, MODE>(A, B, n, row0, col0, smem, cr, ci, dr, tid, wr, wc, lr, lg);
   } else {
     using C = typename CT<R>::C;
     C* sA = reinterpret_cast<C*>(smem);
@@ -331,7 +414,7 @@ __device__ __forceinline__ void term(const typename CT<R>::C* __restrict__ A, co
 }
 
 // (R1, R2, M1) of the FP32 / FP64 panels -> (Re C, Im C) in cr, ci; the 3xBF16 panels
-// accumulate Re C, Im C directly.
+// leave (Re+, Im, Re-) -> Re C = Re+ - Re-.
 template <int TILE, typename R, int P>
 __device__ __forceinline__ void finish_terms(typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
                                              typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
@@ -344,6 +427,11 @@ __device__ __forceinline__ void finish_terms(typename CT<R>::V4 (&cr)[TILE / 32]
         ci[p][q] -= cr[p][q] + dr[p][q];
         cr[p][q] -= dr[p][q];
       }
+  } else {
+#pragma unroll
+    for (int p = 0; p < TILE / 32; ++p)
+#pragma unroll
+      for (int q = 0; q < TILE / 32; ++q) cr[p][q] -= dr[p][q];
   }
 }
 
