@@ -128,6 +128,43 @@ def test_dense512_full_batch(prec, tol):
     assert err0 < 1e-12
 
 
+def test_dense512_batch1_full_length():
+    """The n = 512 system with 16 dense jump operators at batch 1 over the full 10000 RK4
+    steps (dt = 0.01) in FP32 and 3xBF16 against the FP64-MFMA GPU path: populations (the
+    diagonal) at every 1000th step within the 1e-6 population gate, the final state within
+    5e-5 of max |rho|, trace within 1e-4 and Hermiticity within 1e-6.  The trace gate is the
+    loosest: the FP32 accumulation of 512-term sums leaves each diagonal element of the
+    generator ~1e-8 off, and with a slowly varying state those errors repeat from step to
+    step, so Tr rho drifts linearly (FP32: 2.2e-5 after 10000 steps, ~4e-8 per population);
+    trace renormalisation removes the drift where a caller needs Tr rho = 1."""
+    from lime_amd import configs
+    from lime_amd.propagator import LindbladPropagator
+    cfg = configs.dense_single(batch=1)
+    nt, nout, dt = cfg["nt"], 1000, cfg["dt"]
+    assert nt == 10000
+    out = {}
+    for p in ("fp64", "fp32", "3xbf16"):
+        print(f"[fullsize] n=512 batch 1 {p}, {nt} steps", flush=True)
+        prop = LindbladPropagator(cfg["H"], cfg["c_ops"], precision=p)
+        rho, _, states = prop.run(cfg["rho0"], dt, nt, nout=nout, store_states=True)
+        torch.cuda.synchronize()
+        out[p] = (rho[0], torch.diagonal(states[0], dim1=-2, dim2=-1))
+    ref_rho, ref_pop = out["fp64"]
+    scale = float(torch.max(torch.abs(ref_rho)))
+    for p in ("fp32", "3xbf16"):
+        rho, pop = out[p]
+        perr = float(torch.max(torch.abs(pop - ref_pop)))
+        serr = float(torch.max(torch.abs(rho - ref_rho))) / scale
+        tr = abs(complex(torch.trace(rho).item()) - 1)
+        herm = float(torch.max(torch.abs(rho - rho.conj().T)))
+        print(f"[fullsize] n=512 batch 1 {p} vs fp64 GPU over {nt} steps: populations {perr:.3e}, "
+              f"state {serr:.3e} rel, trace {tr:.3e}, hermiticity {herm:.3e}", flush=True)
+        assert perr < 1e-6
+        assert serr < 5e-5
+        assert tr < 1e-4
+        assert herm < 1e-6
+
+
 def test_gradient_full_batch():
     """The gradient workload at full size (256 chains of 128 sites + ground state, n = 129,
     2000 RK4 steps, checkpoints every 64), the dephasing rates given per site (all equal to
