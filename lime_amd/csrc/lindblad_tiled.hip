@@ -268,6 +268,17 @@ template <int ROWS, int COLS>
 __device__ __forceinline__ void load_pairs(const float2* __restrict__ M, int n, int r0, int c0,
                                            float2v (&reg)[2 * PanelPairs<ROWS, COLS>::NP], int tid) {
   const float2v* __restrict__ Mv = reinterpret_cast<const float2v*>(M);
+  if (r0 + ROWS <= n && c0 + COLS <= n) {     // interior panel (uniform): no bounds tests
+#pragma unroll
+    for (int i = 0; i < PanelPairs<ROWS, COLS>::NP; ++i) {
+      const int q = tid + 256 * i;
+      const int r = q / (COLS / 2), c = 2 * (q % (COLS / 2));
+      const float2v* p = Mv + (size_t)(r0 + r) * n + c0 + c;
+      reg[2 * i] = p[0];
+      reg[2 * i + 1] = p[1];
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < PanelPairs<ROWS, COLS>::NP; ++i) {
     const int q = tid + 256 * i;
@@ -388,6 +399,11 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
       }
   }
 }
+
+// Waves per SIMD the compiler must leave room for: the 64 x 64 3xBF16 kernels would take
+// more than 168 VGPRs with the interior-panel load path and drop to 2 waves per SIMD.
+template <int TILE, int P>
+constexpr int kMinWaves = (TILE == 64 && P == 1) ? 3 : 1;
 
 // LDS bytes of a GEMM-staging area.
 template <int TILE, typename R, int P>
@@ -522,7 +538,7 @@ __device__ __forceinline__ void left_tile(const TiledArgs& a, int b, int k, cons
 
 // T[b][k] = L[set][k] @ S[b]  for every dense operator k.
 template <int TILE, typename R, int P>
-__global__ void __launch_bounds__(256) lindblad_left_tiled(TiledArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves<TILE, P>))) lindblad_left_tiled(TiledArgs a) {
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
   const int per = tiles_per_dim(a.n, TILE) * tiles_per_dim(a.n, TILE);
   const int blk = xcd_block();
@@ -680,7 +696,7 @@ __device__ __forceinline__ void border_index(int nc, int oidx, int& bi, int& bj)
 
 // One RK4 stage for every operator of the batch.
 template <int TILE, typename R, int P>
-__global__ void __launch_bounds__(256) lindblad_stage_tiled(TiledArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves<TILE, P>))) lindblad_stage_tiled(TiledArgs a) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
