@@ -204,8 +204,11 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
 //   NEG_AH (C -= a conj(h))    B = (-hx, hy):  Re += ax (-hx) - ay hy,  Im += ax hy + ay (-hx)
 //   I_AH   (C += i a conj(h))  B = (hx, hy):   Re += ax hy - ay hx,  Im += ax hx + ay hy
 constexpr int KB = 32;          // K-panel width (complex elements) of the 3xBF16 staging
-constexpr int PP = KB + 8;      // BF16 pitch of [row][k] planes (80-byte rows: 16 rows of a
-                                // fragment read hit distinct banks)
+constexpr int PP = KB + 16;     // BF16 pitch of [row][k] planes: 96-byte rows, so the 16-byte
+                                // fragment reads of 8 consecutive rows hit distinct banks and
+                                // the staging stores of 4 rows overlap on fewer banks than at
+                                // 80 bytes (dense512: +3.5 %; 64-byte rows with an XOR
+                                // swizzle: +0.6 %); the 64-tile's 48 KB still fits 3 per CU
 
 typedef unsigned short u16;
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
