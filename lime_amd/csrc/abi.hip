@@ -148,8 +148,8 @@ int rk4_impl(const lime_lindblad_batch* d, const lime_step_params* p, void* stre
   }
 
   // ---- tiled path: one launch per RK4 stage on the workspace
-  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, d->flags, is_fp64(d),
-                                                 d->obs ? d->n_eops : 0);
+  const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->group, d->n_dense, d->flags, is_fp64(d),
+                                                 d->obs ? d->n_eops : 0, d->rops != nullptr);
   if (noalloc && need > kScratchBytes)
     return fail(LIME_ERR_ARG, std::string(name) + ": this batch needs " + std::to_string(need) +
                                   " bytes of scratch, above the static " + std::to_string(kScratchBytes) +
@@ -214,8 +214,8 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
   if (use_resident(d)) return 0;
-  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->n_dense, d->flags, is_fp64(d),
-                                             d->obs ? d->n_eops : 0);
+  return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->group, d->n_dense, d->flags, is_fp64(d),
+                                             d->obs ? d->n_eops : 0, d->rops != nullptr);
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {

@@ -306,18 +306,79 @@ __device__ __forceinline__ void store_pairs(u16* sM, const float2v (&reg)[2 * Pa
   }
 }
 
-template <int TILE, int MODE>
-__device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, const float2* __restrict__ Bsrc, int n,
+// Pre-split operands (the constant H_eff, L_k, R_k): four BF16 planes of n rows x ldp
+// (n rounded up to 8, zero beyond n), so a ROWS x KB panel of [row][k] planes is staged as
+// 16-byte chunks (8 consecutive k of one row of one plane), NC chunks per thread.
+template <int ROWS>
+struct PanelChunks {
+  static constexpr int PER_PLANE = ROWS * KB / 8;
+  static constexpr int NC = 4 * PER_PLANE / 256;
+};
+template <int ROWS>
+__device__ __forceinline__ void load_chunks(const u16* __restrict__ M, int n, int ldp, int r0, int k0,
+                                            u16x8 (&reg)[PanelChunks<ROWS>::NC], int tid) {
+  const size_t plane = (size_t)n * ldp;
+#pragma unroll
+  for (int i = 0; i < PanelChunks<ROWS>::NC; ++i) {
+    const int q = tid + 256 * i;
+    const int pl = q / PanelChunks<ROWS>::PER_PLANE, e = q % PanelChunks<ROWS>::PER_PLANE;
+    const int gr = r0 + e / (KB / 8), gk = k0 + 8 * (e % (KB / 8));
+    u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (gr < n && gk < ldp) v = *reinterpret_cast<const u16x8*>(M + pl * plane + (size_t)gr * ldp + gk);
+    reg[i] = v;
+  }
+}
+// NEG negates the real parts (planes 0 and 1)
+template <int ROWS, bool NEG>
+__device__ __forceinline__ void store_chunks(u16* sM, const u16x8 (&reg)[PanelChunks<ROWS>::NC], int tid) {
+#pragma unroll
+  for (int i = 0; i < PanelChunks<ROWS>::NC; ++i) {
+    const int q = tid + 256 * i;
+    const int pl = q / PanelChunks<ROWS>::PER_PLANE, e = q % PanelChunks<ROWS>::PER_PLANE;
+    u16x8 v = reg[i];
+    if (NEG && pl < 2) v ^= (u16)0x8000;
+    *reinterpret_cast<u16x8*>(sM + pl * ROWS * PP + (e / (KB / 8)) * PP + 8 * (e % (KB / 8))) = v;
+  }
+}
+
+// One ROWS x KB [row][k] panel held in registers between its load and its LDS store: from a
+// complex64 operand (split while staged) or from pre-split planes (PRE).
+template <int ROWS, bool PRE>
+struct RowPanel {
+  float2v r[2 * PanelPairs<ROWS, KB>::NP];
+  __device__ __forceinline__ void load(const void* M, int n, int, int r0, int k0, int tid) {
+    load_pairs<ROWS, KB>(static_cast<const float2*>(M), n, r0, k0, r, tid);
+  }
+  template <bool NEG>
+  __device__ __forceinline__ void store(u16* sM, int tid) const {
+    store_pairs<ROWS, KB, PP, NEG>(sM, r, tid);
+  }
+};
+template <int ROWS>
+struct RowPanel<ROWS, true> {
+  u16x8 r[PanelChunks<ROWS>::NC];
+  __device__ __forceinline__ void load(const void* M, int n, int ldp, int r0, int k0, int tid) {
+    load_chunks<ROWS>(static_cast<const u16*>(M), n, ldp, r0, k0, r, tid);
+  }
+  template <bool NEG>
+  __device__ __forceinline__ void store(u16* sM, int tid) const {
+    store_chunks<ROWS, NEG>(sM, r, tid);
+  }
+};
+
+// APS / BPS: the A operand / the B operand of a conjugate-transpose mode comes pre-split
+// (planes with row pitch ldp); otherwise complex64, split while staged.
+template <int TILE, int MODE, bool APS, bool BPS>
+__device__ __forceinline__ void gemm_term_bf3(const void* __restrict__ A, const void* __restrict__ Bsrc, int n, int ldp,
                                               int row0, int col0, unsigned char* smem,
                                               floatx4 (&cr)[TILE / 32][TILE / 32],
                                               floatx4 (&ci)[TILE / 32][TILE / 32],
                                               floatx4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
                                               int lr, int lg) {
+  static_assert(!(BPS && MODE == PLAIN), "the PLAIN B operand is staged as [k][column] planes");
   constexpr int BW = TILE / 32;
   constexpr int PB = pb_pitch<TILE>();
   constexpr int APLANE = TILE * PP;
-  constexpr int NA = 2 * PanelPairs<TILE, KB>::NP;                       // A: TILE rows x KB
-  constexpr int NB = 2 * (MODE == PLAIN ? PanelPairs<KB, TILE>::NP : PanelPairs<TILE, KB>::NP);
   bool rlive[BW], clive[BW];
 #pragma unroll
   for (int p = 0; p < BW; ++p) {
@@ -331,19 +392,21 @@ __device__ __forceinline__ void gemm_term_bf3(const float2* __restrict__ A, cons
   // Measured at n = 512: +11 % for 32 x 32 tiles; for 64 x 64 tiles the 32 prefetch VGPRs
   // cost a wave per SIMD (3 -> 2) and loading each panel when it is staged is 8 % faster.
   constexpr bool PF = TILE == 32;
-  float2v ra[NA], rb[NB];
+  RowPanel<TILE, APS> ra;
+  RowPanel<TILE, BPS> rb;                     // B rows (conjugate-transpose modes)
+  float2v rp[2 * PanelPairs<KB, TILE>::NP];   // B = KB rows of TILE columns (PLAIN)
   auto load_panel = [&](int k) {
-    load_pairs<TILE, KB>(A, n, row0, k, ra, tid);
-    if constexpr (MODE == PLAIN) load_pairs<KB, TILE>(Bsrc, n, k, col0, rb, tid);
-    else load_pairs<TILE, KB>(Bsrc, n, col0, k, rb, tid);
+    ra.load(A, n, ldp, row0, k, tid);
+    if constexpr (MODE == PLAIN) load_pairs<KB, TILE>(static_cast<const float2*>(Bsrc), n, k, col0, rp, tid);
+    else rb.load(Bsrc, n, ldp, col0, k, tid);
   };
   if constexpr (PF) load_panel(0);
   for (int k0 = 0; k0 < n; k0 += KB) {
     if constexpr (!PF) load_panel(k0);
     __syncthreads();                          // previous panel fully consumed
-    store_pairs<TILE, KB, PP, false>(sA, ra, tid);
-    if constexpr (MODE == PLAIN) store_pairs<KB, TILE, PB, false>(sB, rb, tid);
-    else store_pairs<TILE, KB, PP, MODE == NEG_AH>(sB, rb, tid);
+    ra.template store<false>(sA, tid);
+    if constexpr (MODE == PLAIN) store_pairs<KB, TILE, PB, false>(sB, rp, tid);
+    else rb.template store<MODE == NEG_AH>(sB, tid);
     __syncthreads();
     if constexpr (PF) {
       if (k0 + KB < n) load_panel(k0 + KB);   // prefetch the next panel into registers
@@ -416,19 +479,27 @@ constexpr int smem_bytes() {
 }
 
 // One product term in the selected arithmetic.
-template <int TILE, int MODE, typename R, int P>
-__device__ __forceinline__ void term(const typename CT<R>::C* __restrict__ A, const typename CT<R>::C* __restrict__ B,
-                                     int n, int row0, int col0, unsigned char* smem,
+// T_k = L_k S between the two kernels of a stage: 3xBF16 32-tiles write it as split planes
+// (dense512 batch 1: +16 %); the 64-tiles keep complex64 (their 4x more 2-byte plane stores
+// cost 2.6 % at batch 32).
+template <int TILE, int P>
+constexpr bool tk_split() { return P == 1 && TILE == 32; }
+
+// (3xBF16: APS / BPS mark pre-split operands, with row pitch ldp.)
+template <int TILE, int MODE, typename R, int P, bool APS = false, bool BPS = false>
+__device__ __forceinline__ void term(const void* __restrict__ A, const void* __restrict__ B, int n, int ldp, int row0,
+                                     int col0, unsigned char* smem,
                                      typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
                                      typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
                                      typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
                                      int lr, int lg) {
   if constexpr (P == 1) {
-    gemm_term_bf3<TILE, MODE>(A, B, n, row0, col0, smem, cr, ci, dr, tid, wr, wc, lr, lg);
+    gemm_term_bf3<TILE, MODE, APS, BPS>(A, B, n, ldp, row0, col0, smem, cr, ci, dr, tid, wr, wc, lr, lg);
   } else {
     using C = typename CT<R>::C;
     C* sA = reinterpret_cast<C*>(smem);
-    gemm_term<TILE, MODE, R>(A, B, n, row0, col0, sA, sA + TILE * KWP, cr, ci, dr, tid, wr, wc, lr, lg);
+    gemm_term<TILE, MODE, R>(static_cast<const C*>(A), static_cast<const C*>(B), n, row0, col0, sA,
+                             sA + TILE * KWP, cr, ci, dr, tid, wr, wc, lr, lg);
   }
 }
 
@@ -495,14 +566,27 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
   for (int p = 0; p < TILE / 32; ++p)
 #pragma unroll
     for (int q = 0; q < TILE / 32; ++q) dr[p][q] = V4{0, 0, 0, 0};
-  term<TILE, PLAIN, R, P>(H, S, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+  // 3xBF16: H_eff, R_k (pre-split once per call) and, for 32-tiles, T_k (written split)
+  // from planes
+  constexpr bool PS = P == 1;
+  const size_t np = (size_t)4 * n * a.ldp;   // u16 elements of one pre-split matrix
+  const void* Hop = PS ? static_cast<const void*>(a.hsp + (size_t)set * np) : static_cast<const void*>(H);
+  term<TILE, PLAIN, R, P, PS, false>(Hop, S, n, a.ldp, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr,
+                                     g.lg);
   // one-sided generators (dX/dt = G X, H_eff = i G) have no X H_eff^dag product
   if (!(a.flags & kFlagOneSided))
-    term<TILE, NEG_AH, R, P>(S, H, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+    term<TILE, NEG_AH, R, P, false, PS>(S, Hop, n, a.ldp, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc,
+                                        g.lr, g.lg);
   for (int k = 0; k < a.n_dense; ++k) {
-    const C* T = static_cast<const C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
-    const C* Rk = static_cast<const C*>(a.rops ? a.rops : a.lops) + ((size_t)set * a.n_dense + k) * nn;
-    term<TILE, I_AH, R, P>(T, Rk, n, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+    constexpr bool TS = tk_split<TILE, P>();
+    const size_t tk = (size_t)b * a.n_dense + k;
+    const void* T = TS ? static_cast<const void*>(static_cast<const u16*>(a.tbuf) + tk * np)
+                       : static_cast<const void*>(static_cast<const C*>(a.tbuf) + tk * nn);
+    const size_t rk = (size_t)set * a.n_dense + k;
+    const void* Rk = PS ? static_cast<const void*>((a.rsp ? a.rsp : a.lsp) + rk * np)
+                        : static_cast<const void*>(static_cast<const C*>(a.rops ? a.rops : a.lops) + rk * nn);
+    term<TILE, I_AH, R, P, TS, PS>(T, Rk, n, a.ldp, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr,
+                                   g.lg);
   }
   finish_terms<TILE, R, P>(cr, ci, dr);
 }
@@ -522,21 +606,50 @@ __device__ __forceinline__ void left_tile(const TiledArgs& a, int b, int k, cons
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = dr[p][q] = V4{0, 0, 0, 0};
-  term<TILE, PLAIN, R, P>(static_cast<const C*>(a.lops) + ((size_t)set * a.n_dense + k) * nn,
-                          static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, g.row0, g.col0, smem, cr, ci,
-                          dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+  // 3xBF16: L_k from its pre-split planes
+  constexpr bool PS = P == 1;
+  const size_t lk = (size_t)set * a.n_dense + k;
+  const void* Lk = PS ? static_cast<const void*>(a.lsp + lk * 4 * a.n * a.ldp)
+                      : static_cast<const void*>(static_cast<const C*>(a.lops) + lk * nn);
+  term<TILE, PLAIN, R, P, PS, false>(Lk, static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, a.ldp, g.row0,
+                                     g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
   finish_terms<TILE, R, P>(cr, ci, dr);
-  C* T = static_cast<C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
+  if constexpr (tk_split<TILE, P>()) {
+    // 3xBF16, 32-tiles: T_k goes out as split planes (row pitch ldp; the columns n .. ldp - 1
+    // hold the products with the zero columns beyond n, i.e. zeros)
+    const size_t plane = (size_t)a.n * a.ldp;
+    u16* T = static_cast<u16*>(a.tbuf) + ((size_t)b * a.n_dense + k) * 4 * plane;
 #pragma unroll
-  for (int p = 0; p < BW; ++p)
+    for (int p = 0; p < BW; ++p)
 #pragma unroll
-    for (int q = 0; q < BW; ++q)
+      for (int q = 0; q < BW; ++q)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
-        const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-        if (i < a.n && j < a.n) T[(size_t)i * a.n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
-      }
+        for (int v = 0; v < 4; ++v) {
+          const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+          const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+          if (i < a.n && j < a.ldp) {
+            unsigned h, l;
+            split2(cr[p][q][v], ci[p][q][v], h, l);
+            u16* t = T + (size_t)i * a.ldp + j;
+            t[0] = (u16)h;
+            t[plane] = (u16)l;
+            t[2 * plane] = (u16)(h >> 16);
+            t[3 * plane] = (u16)(l >> 16);
+          }
+        }
+  } else {
+    C* T = static_cast<C*>(a.tbuf) + ((size_t)b * a.n_dense + k) * nn;
+#pragma unroll
+    for (int p = 0; p < BW; ++p)
+#pragma unroll
+      for (int q = 0; q < BW; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+          const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+          if (i < a.n && j < a.n) T[(size_t)i * a.n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
+        }
+  }
 }
 
 // T[b][k] = L[set][k] @ S[b]  for every dense operator k.
@@ -1249,8 +1362,12 @@ __global__ void init_stage_operand(const double2* __restrict__ rho, C* __restric
 
 // Byte offsets of the workspace regions (see tiled_workspace_bytes).
 struct WorkLayout {
-  size_t s0, s1, inc, tbuf, tscale, obs_part, total;
+  size_t s0, s1, inc, tbuf, tscale, obs_part, split, total;
+  int ldp, nsets;
 };
+
+// Row pitch of the pre-split planes: n rounded up to whole 16-byte chunks of BF16.
+static int split_pitch(int n) { return (n + 7) & ~7; }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -1264,7 +1381,8 @@ static int stage_wgs(int n, int n_dense, int flags, int tile) {
   return layout(t, tile);
 }
 
-static WorkLayout work_layout(int batch, int n, int n_dense, int flags, bool fp64, int n_eops) {
+static WorkLayout work_layout(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
+                              bool rops) {
   const bool renorm = (flags & 1) != 0;
   const size_t op = (size_t)batch * n * n * (fp64 ? sizeof(double2) : sizeof(float2));
   WorkLayout w;
@@ -1272,17 +1390,27 @@ static WorkLayout work_layout(int batch, int n, int n_dense, int flags, bool fp6
   w.s1 = align256(op);
   w.inc = w.s1 + align256(op);
   w.tbuf = w.inc + align256(op);
-  w.tscale = w.tbuf + align256(op * (size_t)n_dense);
+  // T_k: complex64 / complex128, or (3xBF16) split planes of row pitch split_pitch(n)
+  const bool split = !fp64 && (flags & 0x30) == 0x10;
+  const size_t tsz = split ? (size_t)batch * n_dense * 4 * n * split_pitch(n) * sizeof(unsigned short)
+                           : op * (size_t)n_dense;
+  w.tscale = w.tbuf + align256(tsz);
   w.obs_part = w.tscale + (renorm ? align256((size_t)batch * sizeof(double)) : 0);
   // slots for the larger of the two tile choices (32-tiles: more workgroups)
   const int w32 = stage_wgs(n, n_dense, flags, 32), w64 = stage_wgs(n, n_dense, flags, 64);
   const int slots = w32 > w64 ? w32 : w64;
-  w.total = w.obs_part + (n_eops > 0 ? align256((size_t)batch * slots * n_eops * sizeof(double2)) : 0);
+  w.split = w.obs_part + (n_eops > 0 ? align256((size_t)batch * slots * n_eops * sizeof(double2)) : 0);
+  // 3xBF16: the pre-split planes of H_eff, L_k (and R_k) of every operator set
+  w.ldp = split_pitch(n);
+  w.nsets = (batch + (group > 1 ? group : 1) - 1) / (group > 1 ? group : 1);
+  const size_t mats = (size_t)w.nsets * (1 + (size_t)n_dense * (rops ? 2 : 1));
+  w.total = w.split + (split ? align256(mats * 4 * n * w.ldp * sizeof(unsigned short)) : 0);
   return w;
 }
 
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, int flags, bool fp64, int n_eops) {
-  return work_layout(batch, n, n_dense, flags, fp64, n_eops).total;
+size_t tiled_workspace_bytes(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
+                             bool rops) {
+  return work_layout(batch, n, group, n_dense, flags, fp64, n_eops, rops).total;
 }
 
 int tiled_pick(int batch, int n) {
@@ -1290,13 +1418,60 @@ int tiled_pick(int batch, int n) {
   return ((size_t)batch * t64 * t64 >= 512) ? 64 : 32;
 }
 
+// Pre-split planes of count n x n complex64 matrices: thread t forms one 16-byte chunk (8
+// consecutive columns of one row) of each of the four planes, zero beyond column n.
+__global__ void __launch_bounds__(256) split_planes(const float2* __restrict__ src, unsigned short* __restrict__ dst,
+                                                    int n, int ldp, size_t chunks) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= chunks) return;
+  const int cpr = ldp / 8;
+  const size_t row = t / cpr;                 // matrix * n + row within the matrix
+  const int k0 = 8 * (int)(t % cpr);
+  const size_t m = row / n, r = row % n;
+  const float2* s = src + row * n;
+  unsigned xh[4], xl[4], yh[4], yl[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int k = k0 + 2 * e;
+    const float2 v0 = k < n ? s[k] : make_float2(0.f, 0.f);
+    const float2 v1 = k + 1 < n ? s[k + 1] : make_float2(0.f, 0.f);
+    split2(v0.x, v1.x, xh[e], xl[e]);
+    split2(v0.y, v1.y, yh[e], yl[e]);
+  }
+  const size_t plane = (size_t)n * ldp;
+  unsigned short* d = dst + m * 4 * plane + r * ldp + k0;
+  *reinterpret_cast<uint4*>(d) = make_uint4(xh[0], xh[1], xh[2], xh[3]);
+  *reinterpret_cast<uint4*>(d + plane) = make_uint4(xl[0], xl[1], xl[2], xl[3]);
+  *reinterpret_cast<uint4*>(d + 2 * plane) = make_uint4(yh[0], yh[1], yh[2], yh[3]);
+  *reinterpret_cast<uint4*>(d + 3 * plane) = make_uint4(yl[0], yl[1], yl[2], yl[3]);
+}
+
 template <typename R, int P>
 static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, void* work, hipStream_t stream,
                           int tile) {
   using C = typename CT<R>::C;
   const size_t nn = (size_t)a.n * a.n;
-  const WorkLayout wl = work_layout(a.batch, a.n, a.n_dense, a.flags, sizeof(R) == 8, a.obs ? a.n_eops : 0);
+  const WorkLayout wl = work_layout(a.batch, a.n, a.group, a.n_dense, a.flags, sizeof(R) == 8, a.obs ? a.n_eops : 0,
+                                    a.rops != nullptr);
   unsigned char* base = static_cast<unsigned char*>(work);
+  if constexpr (P == 1) {
+    // split the constant operators once per call: H_eff, then L_k, then R_k
+    unsigned short* sp = reinterpret_cast<unsigned short*>(base + wl.split);
+    const size_t np = (size_t)4 * a.n * wl.ldp;
+    a.ldp = wl.ldp;
+    a.hsp = sp;
+    a.lsp = sp + (size_t)wl.nsets * np;
+    a.rsp = a.rops ? a.lsp + (size_t)wl.nsets * a.n_dense * np : nullptr;
+    unsigned short* dst[3] = {sp, const_cast<unsigned short*>(a.lsp), const_cast<unsigned short*>(a.rsp)};
+    const void* src[3] = {a.heff, a.lops, a.rops};
+    const int count[3] = {wl.nsets, wl.nsets * a.n_dense, a.rops ? wl.nsets * a.n_dense : 0};
+    for (int m = 0; m < 3; ++m) {
+      const size_t chunks = (size_t)count[m] * a.n * (wl.ldp / 8);
+      if (chunks == 0) continue;
+      hipLaunchKernelGGL(split_planes, dim3((unsigned)((chunks + 255) / 256)), dim3(256), 0, stream,
+                         static_cast<const float2*>(src[m]), dst[m], a.n, wl.ldp, chunks);
+    }
+  }
   C* s0 = reinterpret_cast<C*>(base + wl.s0);
   C* s1 = reinterpret_cast<C*>(base + wl.s1);
   a.inc = base + wl.inc;

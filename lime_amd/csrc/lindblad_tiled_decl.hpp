@@ -16,6 +16,12 @@ struct TiledArgs {
   const void* dmat;
   const void* lops;
   const void* rops;        // right factors of the sandwich terms (NULL: R = L)
+  // 3xBF16: H_eff, L_k and R_k pre-split into BF16 planes (Re hi, Re lo, Im hi, Im lo) of n
+  // rows x ldp (n rounded up to 8, zero beyond n), one set of planes per matrix
+  const unsigned short* hsp;
+  const unsigned short* lsp;
+  const unsigned short* rsp;  // NULL: R = L
+  int ldp;
   const double2* eops_t;
   double2* rho;
   double2* obs;
@@ -44,9 +50,11 @@ struct TiledArgs {
 enum class TiledPrec { FP32 = 0, SPLIT_BF16 = 1, FP64 = 2 };
 
 // Workspace of lime_lindblad_rk4's per-stage path: S0, S1, the increment, T_k (operand
-// precision), the trace scales (renormalisation) and the per-workgroup observable partials
-// (n_eops > 0), each region 256-byte aligned.
-size_t tiled_workspace_bytes(int batch, int n, int n_dense, int flags, bool fp64, int n_eops);
+// precision; 3xBF16: BF16 planes), the trace scales (renormalisation), the per-workgroup
+// observable partials (n_eops > 0) and, for 3xBF16, the pre-split planes of H_eff, L_k and
+// R_k (one set per operator set, ceil(batch / group) sets), each region 256-byte aligned.
+size_t tiled_workspace_bytes(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
+                             bool rops);
 
 // lime_lindblad_batch.flags bit of one-sided generators (LIME_F_ONE_SIDED)
 constexpr int kFlagOneSided = 0x200;

@@ -91,7 +91,9 @@ def _a256(x):
 
 def test_workspace_query(lib):
     """Regions S0, S1, increment, T_k (operand precision), trace scales, observable
-    partials (one slot per stage workgroup of the 32-tile layout), each 256-byte aligned."""
+    partials (one slot per stage workgroup of the 32-tile layout), and for 3xBF16 the
+    pre-split BF16 planes of H_eff, L_k (and R_k) per operator set (4 planes of n rows of
+    n rounded up to 8; T_k in the same planes), each region 256-byte aligned."""
     d = _lib.LindbladBatch()
     d.batch, d.n, d.group, d.eop_group = 4, 64, 1, 1
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 0          # resident
@@ -101,9 +103,17 @@ def test_workspace_query(lib):
     op = _a256(4 * 130 * 130 * 8)
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * op + _a256(2 * 4 * 130 * 130 * 8)
     d.flags, d.n, d.n_dense = _lib.LIME_F_PREC_3XBF16, 16, 0                 # complex64 workspace
-    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 16 * 16 * 8)
+    planes = _a256(4 * 4 * 16 * 16 * 2)                                      # 4 sets of H_eff planes
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 16 * 16 * 8) + planes
     d.flags = _lib.LIME_F_PREC_3XBF16 | _lib.LIME_F_RENORM_TRACE             # + trace scales
-    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 16 * 16 * 8) + 256
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == 3 * _a256(4 * 16 * 16 * 8) + 256 + planes
+    # two operator sets (group 2), n = 20 (row pitch 24), two dense L and separate R_k
+    d.flags, d.n, d.n_dense, d.group = _lib.LIME_F_PREC_3XBF16, 20, 2, 2
+    d.rops = 256                                                             # never dereferenced
+    op = _a256(4 * 20 * 20 * 8)
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == \
+        3 * op + _a256(4 * 2 * 4 * 20 * 24 * 2) + _a256(2 * (1 + 2 * 2) * 4 * 20 * 24 * 2)
+    d.rops, d.group = None, 1
     # observables: n = 129 (bordered 32-tiles: 16 tiles + 9 border workgroups per operator)
     d.flags, d.n, d.n_dense, d.n_eops = _lib.LIME_F_PREC_FP64, 129, 0, 3
     d.obs = 256                                                              # never dereferenced
