@@ -206,8 +206,10 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
 
 // ---- 3xBF16 split products
 // Every real operand is split x = hi + lo (v_cvt_pk_bf16_f32, round to nearest even; x - hi
-// is exact in FP32) as it is staged into LDS, each part in its own plane (Re hi, Re lo,
-// Im hi, Im lo), K-panels of KB = 32 complex elements (one 16x16x32 MFMA k-step).  A rows,
+// is exact in FP32), each part in its own plane (Re hi, Re lo, Im hi, Im lo): the constant
+// H_eff, L_k, R_k once per call into workspace planes (split_planes), T_k by the 32-tile
+// left kernel that produces it, the stage operand (and the 64-tiles' T_k) while it is staged
+// into LDS.  K-panels of KB = 32 complex elements (one 16x16x32 MFMA k-step).  A rows,
 // and the B operand of the conjugate-transpose modes (rows j of M), are planes [row][k]:
 // one 16-byte read per fragment.  The PLAIN B operand (rows k of M) is stored as planes
 // [k][column] and its fragments (8 consecutive k of one column) come through
