@@ -58,12 +58,16 @@ bool use_resident(const lime_lindblad_batch* d) {
   return lime::resident_lds_bytes(d->n, resident_variant(d), d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
 }
 
-// Tiled tile edge requested by d->variant (1 -> 32, 2 -> 64), else the heuristic.
+// Tiled launch shape requested by d->variant: 1 -> 32 x 32 tiles, 2 -> 64 x 64 tiles, 3 ->
+// 32 x 32 tiles with the stage reduction split in kKSplit parts (4x the workgroups for small
+// batches; not with border workgroups); 0 -> the heuristic tile, no split.
+constexpr int kKSplit = 4;
 int tiled_tile(const lime_lindblad_batch* d) {
-  if (d->variant == 1) return 32;
+  if (d->variant == 1 || d->variant == 3) return 32;
   if (d->variant == 2) return 64;
   return lime::tiled_pick(d->batch, d->n);
 }
+int tiled_ksplit(const lime_lindblad_batch* d) { return d->variant == 3 ? kKSplit : 1; }
 
 // ---- static scratch of lime_lindblad_rk4_noalloc: one array per GPU, held until the
 // process ends.  Host code never uses the array's name as an address (on the host it is a
@@ -149,7 +153,8 @@ int rk4_impl(const lime_lindblad_batch* d, const lime_step_params* p, void* stre
 
   // ---- tiled path: one launch per RK4 stage on the workspace
   const size_t need = lime::tiled_workspace_bytes(d->batch, d->n, d->group, d->n_dense, d->flags, is_fp64(d),
-                                                 d->obs ? d->n_eops : 0, d->rops != nullptr);
+                                                 d->obs ? d->n_eops : 0, d->rops != nullptr, tiled_tile(d),
+                                                 tiled_ksplit(d));
   if (noalloc && need > kScratchBytes)
     return fail(LIME_ERR_ARG, std::string(name) + ": this batch needs " + std::to_string(need) +
                                   " bytes of scratch, above the static " + std::to_string(kScratchBytes) +
@@ -181,6 +186,7 @@ int rk4_impl(const lime_lindblad_batch* d, const lime_step_params* p, void* stre
   a.obs = static_cast<double2*>(d->obs);
   a.states = static_cast<double2*>(d->states);
   a.stage_rec = static_cast<double2*>(d->stage_states);
+  a.ks = tiled_ksplit(d);
   const int nout = (d->obs || d->states || d->stage_states) ? p->nout : 0;
   const lime::TiledPrec prec = is_fp64(d) ? lime::TiledPrec::FP64
                               : is_split_bf16(d) ? lime::TiledPrec::SPLIT_BF16 : lime::TiledPrec::FP32;
@@ -208,14 +214,15 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
   probe.variant = 0;
   if (use_resident_bf3(&probe)) return 1;
   if (use_resident(&probe)) return lime::resident_variants(d->n, d->n_dense);
-  return 3;   // tiled: 0 = heuristic, 1 = 32x32 tiles, 2 = 64x64 tiles
+  return 4;   // tiled: 0 = heuristic, 1 = 32x32 tiles, 2 = 64x64 tiles, 3 = K-split 32x32 tiles
 }
 
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {
   if (d == nullptr || d->n < 1 || d->batch < 1) return 0;
   if (use_resident(d)) return 0;
   return (int64_t)lime::tiled_workspace_bytes(d->batch, d->n, d->group, d->n_dense, d->flags, is_fp64(d),
-                                             d->obs ? d->n_eops : 0, d->rops != nullptr);
+                                             d->obs ? d->n_eops : 0, d->rops != nullptr, tiled_tile(d),
+                                             tiled_ksplit(d));
 }
 
 int lime_lindblad_rk4(const lime_lindblad_batch* d, const lime_step_params* p, void* stream) {

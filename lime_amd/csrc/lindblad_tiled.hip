@@ -142,7 +142,7 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
                                           typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
                                           typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
                                           typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32], int tid, int wr,
-                                          int wc, int lr, int lg) {
+                                          int wc, int lr, int lg, int kb, int ke) {
   using C = typename CT<R>::C;
   constexpr int BW = TILE / 32;
   constexpr int NR = TILE * KW / 256;
@@ -154,17 +154,19 @@ __device__ __forceinline__ void gemm_term(const typename CT<R>::C* __restrict__ 
     rlive[p] = row0 + wr * (TILE / 2) + p * 16 < n;
     clive[p] = col0 + wc * (TILE / 2) + p * 16 < n;
   }
+  // the reduction runs over k in [kb, ke): kb a multiple of the panel width, ke = n or a
+  // multiple of it (a K-split part), so no panel straddles two parts
   C ra[NR], rb[NR];
-  load_rows<TILE, KW, C, FAST>(A, n, row0, 0, ra, tid);
-  if (MODE == PLAIN) load_cols<TILE, KW, C, FAST>(Bsrc, n, 0, col0, rb, tid);
-  else load_rows<TILE, KW, C, FAST>(Bsrc, n, col0, 0, rb, tid);
-  for (int k0 = 0; k0 < n; k0 += KW) {
+  load_rows<TILE, KW, C, FAST>(A, n, row0, kb, ra, tid);
+  if (MODE == PLAIN) load_cols<TILE, KW, C, FAST>(Bsrc, n, kb, col0, rb, tid);
+  else load_rows<TILE, KW, C, FAST>(Bsrc, n, col0, kb, rb, tid);
+  for (int k0 = kb; k0 < ke; k0 += KW) {
     __syncthreads();                          // previous panel fully consumed
     store_rows<TILE, KW>(sA, ra, tid);
     if (MODE == PLAIN) store_cols<TILE, KW>(sB, rb, tid);
     else store_rows<TILE, KW>(sB, rb, tid);
     __syncthreads();
-    if (k0 + KW < n) {                        // prefetch the next panel into registers
+    if (k0 + KW < ke) {                       // prefetch the next panel into registers
       load_rows<TILE, KW, C, FAST>(A, n, row0, k0 + KW, ra, tid);
       if (MODE == PLAIN) load_cols<TILE, KW, C, FAST>(Bsrc, n, k0 + KW, col0, rb, tid);
       else load_rows<TILE, KW, C, FAST>(Bsrc, n, col0, k0 + KW, rb, tid);
@@ -392,7 +394,7 @@ __device__ __forceinline__ void gemm_term_bf3(const void* __restrict__ A, const 
                                               floatx4 (&cr)[TILE / 32][TILE / 32],
                                               floatx4 (&ci)[TILE / 32][TILE / 32],
                                               floatx4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
-                                              int lr, int lg) {
+                                              int lr, int lg, int kb, int ke) {
   static_assert(!(BPS && MODE == PLAIN), "the PLAIN B operand is staged as [k][column] planes");
   constexpr int BW = TILE / 32;
   constexpr int PB = pb_pitch<TILE>();
@@ -418,8 +420,8 @@ __device__ __forceinline__ void gemm_term_bf3(const void* __restrict__ A, const 
     if constexpr (MODE == PLAIN) load_pairs<KB, TILE>(static_cast<const float2*>(Bsrc), n, k, col0, rp, tid);
     else rb.load(Bsrc, n, ldp, col0, k, tid);
   };
-  if constexpr (PF) load_panel(0);
-  for (int k0 = 0; k0 < n; k0 += KB) {
+  if constexpr (PF) load_panel(kb);
+  for (int k0 = kb; k0 < ke; k0 += KB) {
     if constexpr (!PF) load_panel(k0);
     __syncthreads();                          // previous panel fully consumed
     ra.template store<false>(sA, tid);
@@ -427,7 +429,7 @@ __device__ __forceinline__ void gemm_term_bf3(const void* __restrict__ A, const 
     else rb.template store<MODE == NEG_AH>(sB, tid);
     __syncthreads();
     if constexpr (PF) {
-      if (k0 + KB < n) load_panel(k0 + KB);   // prefetch the next panel into registers
+      if (k0 + KB < ke) load_panel(k0 + KB);  // prefetch the next panel into registers
     }
     u16x8 a0h[BW], a0l[BW], a1h[BW], a1l[BW], b0h[BW], b0l[BW], b1h[BW], b1l[BW];
 #pragma unroll
@@ -510,9 +512,9 @@ __device__ __forceinline__ void term(const void* __restrict__ A, const void* __r
                                      typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
                                      typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
                                      typename CT<R>::V4 (&dr)[TILE / 32][TILE / 32], int tid, int wr, int wc,
-                                     int lr, int lg) {
+                                     int lr, int lg, int kb, int ke) {
   if constexpr (P == 1) {
-    gemm_term_bf3<TILE, MODE, APS, BPS>(A, B, n, ldp, row0, col0, smem, cr, ci, dr, tid, wr, wc, lr, lg);
+    gemm_term_bf3<TILE, MODE, APS, BPS>(A, B, n, ldp, row0, col0, smem, cr, ci, dr, tid, wr, wc, lr, lg, kb, ke);
   } else {
     using C = typename CT<R>::C;
     C* sA = reinterpret_cast<C*>(smem);
@@ -520,7 +522,7 @@ __device__ __forceinline__ void term(const void* __restrict__ A, const void* __r
     // the 32-tiles (n = 129 gradient forward: 3.6 % slower) and the persistent ragged kernel
     // (9.5 % slower) keep the bounds-tested loads
     gemm_term<TILE, MODE, R, !RAG && TILE == 64>(static_cast<const C*>(A), static_cast<const C*>(B), n, row0, col0, sA,
-                             sA + TILE * KWP, cr, ci, dr, tid, wr, wc, lr, lg);
+                             sA + TILE * KWP, cr, ci, dr, tid, wr, wc, lr, lg, kb, ke);
   }
 }
 
@@ -571,11 +573,12 @@ __device__ __forceinline__ Geo geometry(int n, int tile_index, int tid) {
   return g;
 }
 
-// C = H S - S H^dag + i sum_k T_k R_k^dag for operator b (set = b / group).
+// C = H S - S H^dag + i sum_k T_k R_k^dag for operator b (set = b / group), the reductions
+// over k in [kb, ke) (a K-split part; all of them: kb = 0, ke = n).
 template <int TILE, typename R, int P, bool RAG = false>
 __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const Geo& g, unsigned char* smem,
                                                 typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
-                                                typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32]) {
+                                                typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32], int kb, int ke) {
   using C = typename CT<R>::C;
   const int n = a.n, set = b / a.group;
   const size_t nn = (size_t)n * n;
@@ -593,11 +596,11 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
   const size_t np = (size_t)4 * n * a.ldp;   // u16 elements of one pre-split matrix
   const void* Hop = PS ? static_cast<const void*>(a.hsp + (size_t)set * np) : static_cast<const void*>(H);
   term<TILE, PLAIN, R, P, PS, false, RAG>(Hop, S, n, a.ldp, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr,
-                                     g.lg);
+                                          g.lg, kb, ke);
   // one-sided generators (dX/dt = G X, H_eff = i G) have no X H_eff^dag product
   if (!(a.flags & kFlagOneSided))
     term<TILE, NEG_AH, R, P, false, PS, RAG>(S, Hop, n, a.ldp, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc,
-                                        g.lr, g.lg);
+                                             g.lr, g.lg, kb, ke);
   for (int k = 0; k < a.n_dense; ++k) {
     constexpr bool TS = tk_split<TILE, P>();
     const size_t tk = (size_t)b * a.n_dense + k;
@@ -607,7 +610,7 @@ __device__ __forceinline__ void generator_gemms(const TiledArgs& a, int b, const
     const void* Rk = PS ? static_cast<const void*>((a.rsp ? a.rsp : a.lsp) + rk * np)
                         : static_cast<const void*>(static_cast<const C*>(a.rops ? a.rops : a.lops) + rk * nn);
     term<TILE, I_AH, R, P, TS, PS, RAG>(T, Rk, n, a.ldp, g.row0, g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr,
-                                   g.lg);
+                                        g.lg, kb, ke);
   }
   finish_terms<TILE, R, P>(cr, ci, dr);
 }
@@ -633,7 +636,7 @@ __device__ __forceinline__ void left_tile(const TiledArgs& a, int b, int k, cons
   const void* Lk = PS ? static_cast<const void*>(a.lsp + lk * 4 * a.n * a.ldp)
                       : static_cast<const void*>(static_cast<const C*>(a.lops) + lk * nn);
   term<TILE, PLAIN, R, P, PS, false, RAG>(Lk, static_cast<const C*>(a.s_cur) + (size_t)b * nn, a.n, a.ldp, g.row0,
-                                     g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg);
+                                     g.col0, smem, cr, ci, dr, g.tid, g.wr, g.wc, g.lr, g.lg, 0, a.n);
   finish_terms<TILE, R, P>(cr, ci, dr);
   if constexpr (tk_split<TILE, P>()) {
     // 3xBF16, 32-tiles: T_k goes out as split planes (row pitch ldp; the columns n .. ldp - 1
@@ -831,6 +834,52 @@ __device__ __forceinline__ void border_index(int nc, int oidx, int& bi, int& bj)
   }
 }
 
+// RK4 stage epilogue of one tile: stage_elem on every element of the tile, then the
+// tile's share of the records.  Called by every thread of the workgroup.
+template <int TILE, typename R, int P>
+__device__ __forceinline__ void stage_tile_epilogue(const TiledArgs& a, int b, int set, int idx, const Geo& g,
+                                                    typename CT<R>::V4 (&cr)[TILE / 32][TILE / 32],
+                                                    typename CT<R>::V4 (&ci)[TILE / 32][TILE / 32],
+                                                    double2* sRed) {
+  constexpr int BW = TILE / 32;
+  const int n = a.n, nc = a.nc;
+  const size_t nn = (size_t)n * n;
+  const bool rec = (a.stage == 3) && a.rec >= 0;
+  double2 rfin[BW][BW][4];
+#pragma unroll
+  for (int p = 0; p < BW; ++p)
+#pragma unroll
+    for (int q = 0; q < BW; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        rfin[p][q][v] = make_double2(0.0, 0.0);
+        const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+        const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+        if (i >= nc || j >= nc) continue;
+        rfin[p][q][v] = stage_elem<R>(a, b, set, (size_t)i * n + j, cr[p][q][v], ci[p][q][v]);
+      }
+  if (rec && a.obs) {
+    const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
+    for (int e = 0; e < a.n_eops; ++e) {
+      double pr = 0.0, pi = 0.0;
+#pragma unroll
+      for (int p = 0; p < BW; ++p)
+#pragma unroll
+        for (int q = 0; q < BW; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+            const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+            if (i >= nc || j >= nc) continue;
+            const double2 ev = E0[(size_t)e * nn + (size_t)i * n + j];
+            pr += ev.x * rfin[p][q][v].x - ev.y * rfin[p][q][v].y;
+            pi += ev.x * rfin[p][q][v].y + ev.y * rfin[p][q][v].x;
+          }
+      obs_partial(a, b, idx, pr, pi, e, sRed);
+    }
+  }
+}
+
 // One RK4 stage for every operator of the batch.
 template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves<TILE, P>))) lindblad_stage_tiled(TiledArgs a) {
@@ -880,42 +929,82 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
-  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci);
+  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci, 0, n);
+  stage_tile_epilogue<TILE, R, P>(a, b, set, idx, g, cr, ci, sRed);
+}
 
-  // ---- RK4 stage epilogue
-  double2 rfin[BW][BW][4];
+// K-split stage products (a.ks > 1, layouts without border workgroups): ks consecutive
+// workgroups per tile, part p reducing k in [p klen, (p + 1) klen) into its partial sums;
+// lindblad_stage_sum then finishes the stage.  More workgroups for small batches.
+template <int TILE, typename R, int P>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves<TILE, P>))) lindblad_stage_part(TiledArgs a) {
+  using C = typename CT<R>::C;
+  using V4 = typename CT<R>::V4;
+  __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
+  const int n = a.n;
+  const int tiles = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int blk0 = xcd_block();
+  const int part = blk0 % a.ks, blk = blk0 / a.ks;
+  const int b = blk / tiles, idx = blk % tiles;
+  const size_t nn = (size_t)n * n;
+  const Geo g = geometry<TILE>(n, idx, threadIdx.x);
+  constexpr int BW = TILE / 32;
+  V4 cr[BW][BW], ci[BW][BW];
+#pragma unroll
+  for (int p = 0; p < BW; ++p)
+#pragma unroll
+    for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
+  const int kb = part * a.klen, ke = kb + a.klen < n ? kb + a.klen : n;
+  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci, kb, ke);
+  C* Pp = static_cast<C*>(a.part) + ((size_t)part * a.batch + b) * nn;
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        rfin[p][q][v] = make_double2(0.0, 0.0);
         const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
         const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-        if (i >= nc || j >= nc) continue;
-        rfin[p][q][v] = stage_elem<R>(a, b, set, (size_t)i * n + j, cr[p][q][v], ci[p][q][v]);
+        if (i < n && j < n) Pp[(size_t)i * n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
       }
-  if (rec && a.obs) {
-    const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
-    for (int e = 0; e < a.n_eops; ++e) {
-      double pr = 0.0, pi = 0.0;
+}
+
+// The stage of a K-split launch: the partial sums of every part, in part order, then the
+// same epilogue as lindblad_stage_tiled (one workgroup per tile; no border workgroups).
+template <int TILE, typename R, int P>
+__global__ void __launch_bounds__(256) lindblad_stage_sum(TiledArgs a) {
+  using C = typename CT<R>::C;
+  using V4 = typename CT<R>::V4;
+  __shared__ double2 sRed[4];
+  const int n = a.n;
+  const int tiles = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int blk = xcd_block();
+  const int b = blk / tiles, idx = blk % tiles;
+  const size_t nn = (size_t)n * n;
+  const Geo g = geometry<TILE>(n, idx, threadIdx.x);
+  constexpr int BW = TILE / 32;
+  V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
-      for (int p = 0; p < BW; ++p)
+  for (int p = 0; p < BW; ++p)
 #pragma unroll
-        for (int q = 0; q < BW; ++q)
+    for (int q = 0; q < BW; ++q)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
-            const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-            if (i >= nc || j >= nc) continue;
-            const double2 ev = E0[(size_t)e * nn + (size_t)i * n + j];
-            pr += ev.x * rfin[p][q][v].x - ev.y * rfin[p][q][v].y;
-            pi += ev.x * rfin[p][q][v].y + ev.y * rfin[p][q][v].x;
+      for (int v = 0; v < 4; ++v) {
+        const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
+        const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
+        R xr = 0, xi = 0;
+        if (i < n && j < n) {
+          const C* Pp = static_cast<const C*>(a.part) + (size_t)b * nn + (size_t)i * n + j;
+          for (int part = 0; part < a.ks; ++part) {
+            const C x = Pp[(size_t)part * a.batch * nn];
+            xr += x.x;
+            xi += x.y;
           }
-      obs_partial(a, b, idx, pr, pi, e, sRed);
-    }
-  }
+        }
+        cr[p][q][v] = xr;
+        ci[p][q][v] = xi;
+      }
+  stage_tile_epilogue<TILE, R, P>(a, b, b / a.group, idx, g, cr, ci, sRed);
 }
 
 // One application of the generator to every operator of the batch, with a flexible
@@ -951,7 +1040,7 @@ __global__ void __launch_bounds__(256) lindblad_apply_tiled(TiledArgs a) {
   for (int p = 0; p < BW; ++p)
 #pragma unroll
     for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = V4{0, 0, 0, 0};
-  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci);
+  generator_gemms<TILE, R, P>(a, b, g, smem, cr, ci, 0, n);
 #pragma unroll
   for (int p = 0; p < BW; ++p)
 #pragma unroll
@@ -1057,7 +1146,7 @@ lindblad_ragged_persistent(RaggedArgs r) {
           for (int p = 0; p < BW; ++p)
 #pragma unroll
             for (int q = 0; q < BW; ++q) cr[p][q] = ci[p][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-          generator_gemms<TILE, float, 0, true>(a, 0, g, smem[grp], cr, ci);
+          generator_gemms<TILE, float, 0, true>(a, 0, g, smem[grp], cr, ci, 0, n);
 #pragma unroll
           for (int p = 0; p < BW; ++p)
 #pragma unroll
@@ -1298,6 +1387,8 @@ hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64) {
 template <int TILE, typename R, int P>
 static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) {
   const int per = layout(a, TILE);
+  // K-split: 32-tiles without border workgroups only
+  if (a.ks < 1 || a.border_wgs > 0 || TILE != 32) a.ks = 1;
   for (int st = 0; st < 4; ++st) {
     a.stage = st;
     a.s_cur = (st % 2 == 0) ? s0 : s1;
@@ -1306,7 +1397,15 @@ static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) 
       hipLaunchKernelGGL((lindblad_left_tiled<TILE, R, P>), dim3(per * a.batch * a.n_dense), dim3(256), 0,
                          stream, a);
     }
-    hipLaunchKernelGGL((lindblad_stage_tiled<TILE, R, P>), dim3(per * a.batch), dim3(256), 0, stream, a);
+    bool split = false;
+    if constexpr (TILE == 32) {
+      if (a.ks > 1) {
+        hipLaunchKernelGGL((lindblad_stage_part<TILE, R, P>), dim3(per * a.batch * a.ks), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((lindblad_stage_sum<TILE, R, P>), dim3(per * a.batch), dim3(256), 0, stream, a);
+        split = true;
+      }
+    }
+    if (!split) hipLaunchKernelGGL((lindblad_stage_tiled<TILE, R, P>), dim3(per * a.batch), dim3(256), 0, stream, a);
   }
   return hipGetLastError();
 }
@@ -1383,8 +1482,8 @@ __global__ void init_stage_operand(const double2* __restrict__ rho, C* __restric
 
 // Byte offsets of the workspace regions (see tiled_workspace_bytes).
 struct WorkLayout {
-  size_t s0, s1, inc, tbuf, tscale, obs_part, split, total;
-  int ldp, nsets;
+  size_t s0, s1, inc, tbuf, tscale, obs_part, split, part, total;
+  int ldp, nsets, ks, klen;
 };
 
 // Row pitch of the pre-split planes: n rounded up to whole 16-byte chunks of BF16.
@@ -1402,8 +1501,19 @@ static int stage_wgs(int n, int n_dense, int flags, int tile) {
   return layout(t, tile);
 }
 
+// K-split parts actually used (32-tiles without border workgroups only).
+static int effective_ks(int n, int n_dense, int flags, int tile, int ks) {
+  if (ks <= 1 || tile != 32) return 1;
+  TiledArgs t = {};
+  t.n = n;
+  t.n_dense = n_dense;
+  t.flags = flags;
+  layout(t, tile);
+  return t.border_wgs > 0 ? 1 : ks;
+}
+
 static WorkLayout work_layout(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
-                              bool rops) {
+                              bool rops, int tile, int ks) {
   const bool renorm = (flags & 1) != 0;
   const size_t op = (size_t)batch * n * n * (fp64 ? sizeof(double2) : sizeof(float2));
   WorkLayout w;
@@ -1425,13 +1535,18 @@ static WorkLayout work_layout(int batch, int n, int group, int n_dense, int flag
   w.ldp = split_pitch(n);
   w.nsets = (batch + (group > 1 ? group : 1) - 1) / (group > 1 ? group : 1);
   const size_t mats = (size_t)w.nsets * (1 + (size_t)n_dense * (rops ? 2 : 1));
-  w.total = w.split + (split ? align256(mats * 4 * n * w.ldp * sizeof(unsigned short)) : 0);
+  w.part = w.split + (split ? align256(mats * 4 * n * w.ldp * sizeof(unsigned short)) : 0);
+  // K-split partial sums: ks x batch operators in the operand precision; parts of klen
+  // (a multiple of both panel widths) reduction indices
+  w.ks = effective_ks(n, n_dense, flags, tile, ks);
+  w.klen = ((n + w.ks - 1) / w.ks + 31) / 32 * 32;
+  w.total = w.part + (w.ks > 1 ? align256((size_t)w.ks * op) : 0);
   return w;
 }
 
 size_t tiled_workspace_bytes(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
-                             bool rops) {
-  return work_layout(batch, n, group, n_dense, flags, fp64, n_eops, rops).total;
+                             bool rops, int tile, int ks) {
+  return work_layout(batch, n, group, n_dense, flags, fp64, n_eops, rops, tile, ks).total;
 }
 
 int tiled_pick(int batch, int n) {
@@ -1473,7 +1588,10 @@ static hipError_t run_all(TiledArgs a, int nsteps, int step_offset, int nout, vo
   using C = typename CT<R>::C;
   const size_t nn = (size_t)a.n * a.n;
   const WorkLayout wl = work_layout(a.batch, a.n, a.group, a.n_dense, a.flags, sizeof(R) == 8, a.obs ? a.n_eops : 0,
-                                    a.rops != nullptr);
+                                    a.rops != nullptr, tile, a.ks);
+  a.ks = wl.ks;
+  a.klen = wl.klen;
+  a.part = a.ks > 1 ? static_cast<void*>(static_cast<unsigned char*>(work) + wl.part) : nullptr;
   unsigned char* base = static_cast<unsigned char*>(work);
   if constexpr (P == 1) {
     // split the constant operators once per call: H_eff, then L_k, then R_k

@@ -34,6 +34,10 @@ struct TiledArgs {
   double* tscale;          // per-operator 1 / Re Tr rho of the trace renormalisation
   double2* obs_part;       // [batch][slots][n_eops] per-workgroup partial sums of Tr[E rho]
   int slots;               // workgroups per operator of the stage kernel (= obs_part slots)
+  // K-split stage launches (ks > 1): part p reduces k in [p klen, (p + 1) klen) into
+  // part[p][batch][n][n] (operand precision); lindblad_stage_sum finishes the stage
+  int ks, klen;
+  void* part;
   double2* stage_rec;      // [rec_capacity][3][batch][n][n] FP64 stage operands, or NULL
   // generator application (lindblad_apply_tiled): V = L(s_cur);
   //   y = c_lx V + c_z z (complex128), y64 = copy of y in the operand precision,
@@ -51,10 +55,11 @@ enum class TiledPrec { FP32 = 0, SPLIT_BF16 = 1, FP64 = 2 };
 
 // Workspace of lime_lindblad_rk4's per-stage path: S0, S1, the increment, T_k (operand
 // precision; 3xBF16: BF16 planes), the trace scales (renormalisation), the per-workgroup
-// observable partials (n_eops > 0) and, for 3xBF16, the pre-split planes of H_eff, L_k and
-// R_k (one set per operator set, ceil(batch / group) sets), each region 256-byte aligned.
+// observable partials (n_eops > 0), for 3xBF16 the pre-split planes of H_eff, L_k and R_k
+// (one set per operator set, ceil(batch / group) sets), and with a K-split (ks > 1 for this
+// tile edge) the ks partial sums of the stage operators, each region 256-byte aligned.
 size_t tiled_workspace_bytes(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
-                             bool rops);
+                             bool rops, int tile, int ks);
 
 // lime_lindblad_batch.flags bit of one-sided generators (LIME_F_ONE_SIDED)
 constexpr int kFlagOneSided = 0x200;
