@@ -58,16 +58,16 @@ bool use_resident(const lime_lindblad_batch* d) {
   return lime::resident_lds_bytes(d->n, resident_variant(d), d->n_dense, d->obs ? d->n_eops : 0) <= kLdsLimit;
 }
 
-// Tiled launch shape requested by d->variant: 1 -> 32 x 32 tiles, 2 -> 64 x 64 tiles, 3 ->
-// 32 x 32 tiles with the stage reduction split in kKSplit parts (4x the workgroups for small
-// batches; not with border workgroups); 0 -> the heuristic tile, no split.
-constexpr int kKSplit = 4;
+// Tiled launch shape requested by d->variant: 1 -> 32 x 32 tiles, 2 -> 64 x 64 tiles, 3 / 4
+// -> 32 x 32 tiles with the stage reduction split in 2 / 8 parts (more workgroups for small
+// batches; not with border workgroups; dense512 at batch 1: 3xBF16 fastest with 2 parts,
+// FP32 with 8); 0 -> the heuristic tile, no split.
 int tiled_tile(const lime_lindblad_batch* d) {
-  if (d->variant == 1 || d->variant == 3) return 32;
+  if (d->variant == 1 || d->variant == 3 || d->variant == 4) return 32;
   if (d->variant == 2) return 64;
   return lime::tiled_pick(d->batch, d->n);
 }
-int tiled_ksplit(const lime_lindblad_batch* d) { return d->variant == 3 ? kKSplit : 1; }
+int tiled_ksplit(const lime_lindblad_batch* d) { return d->variant == 3 ? 2 : d->variant == 4 ? 8 : 1; }
 
 // ---- static scratch of lime_lindblad_rk4_noalloc: one array per GPU, held until the
 // process ends.  Host code never uses the array's name as an address (on the host it is a
@@ -214,7 +214,7 @@ int lime_lindblad_variants(const lime_lindblad_batch* d) {
   probe.variant = 0;
   if (use_resident_bf3(&probe)) return 1;
   if (use_resident(&probe)) return lime::resident_variants(d->n, d->n_dense);
-  return 4;   // tiled: 0 = heuristic, 1 = 32x32 tiles, 2 = 64x64 tiles, 3 = K-split 32x32 tiles
+  return 5;   // tiled: 0 = heuristic, 1 = 32x32, 2 = 64x64 tiles, 3 / 4 = 32x32 K-split in 2 / 8
 }
 
 int64_t lime_lindblad_workspace_bytes(const lime_lindblad_batch* d) {

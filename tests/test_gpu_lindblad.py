@@ -504,11 +504,12 @@ def test_tiled_records_are_stored_and_bitwise_reproducible(n, prec, kd):
         assert err < 10 * tol
 
 
+@pytest.mark.parametrize("variant", [3, 4])
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-7), ("3xbf16", 2e-6), ("fp64", 1e-13)])
 @pytest.mark.parametrize("n,kd", [(40, 1), (100, 2)])
-def test_ksplit_stage_matches_oracle(n, kd, prec, tol):
-    """Launch variant 3: the stage products of every 32 x 32 tile split into 4 K ranges of
-    32-multiples (4x the workgroups; at n = 40 the last two ranges are empty), whose partial
+def test_ksplit_stage_matches_oracle(n, kd, prec, tol, variant):
+    """Launch variants 3 and 4: the stage products of every 32 x 32 tile split into 2 or 8 K
+    ranges of 32-multiples (at n = 40 with 8 ranges the last six are empty), whose partial
     sums lindblad_stage_sum adds in part order before the same epilogue: records, stored
     states and trace renormalisation against the oracle."""
     rng = np.random.default_rng(950 + n + kd)
@@ -518,14 +519,14 @@ def test_ksplit_stage_matches_oracle(n, kd, prec, tol):
     X0 = np.stack([rand_dm(rng, n), rand_c(rng, n, n) / n])
     A = [rand_herm(rng, n), rand_c(rng, n, n)]
     dt, nt = 0.02, 6
-    prop = _prop()(H, c_ops, group=2, precision=prec, force_tiled=True, variant=3)
-    assert prop.variants() == 4
+    prop = _prop()(H, c_ops, group=2, precision=prec, force_tiled=True, variant=variant)
+    assert prop.variants() == 5
     rho, obs, states = prop.run(X0, dt, nt, e_ops=A, nout=2, store_states=True)
     for b in range(2):
         ref, ref_obs, ref_states = ol.propagate(H, X0[b], c_ops, dt, nt, A, nout=2, store_states=True)
         scale = np.max(np.abs(ref))
         err = np.max(np.abs(rho[b].cpu().numpy() - ref)) / scale
-        print(f"k-split n={n} kd={kd} {prec} b={b}: rel err {err:.3e}")
+        print(f"k-split v{variant} n={n} kd={kd} {prec} b={b}: rel err {err:.3e}")
         assert err < tol
         assert np.max(np.abs(obs[b].cpu().numpy() - ref_obs)) < 10 * tol * np.max(np.abs(ref_obs))
         assert np.max(np.abs(states[b].cpu().numpy() - np.array(ref_states))) < tol * scale
