@@ -60,8 +60,8 @@ bool use_resident(const lime_lindblad_batch* d) {
 
 // Tiled launch shape requested by d->variant: 1 -> 32 x 32 tiles, 2 -> 64 x 64 tiles, 3 / 4
 // -> 32 x 32 tiles with the stage reduction split in 2 / 8 parts (more workgroups for small
-// batches; not with border workgroups; dense512 at batch 1: 3xBF16 fastest with 2 parts,
-// FP32 with 8); 0 -> the heuristic tile, no split.
+// batches; dense512 at batch 1: 3xBF16 fastest with 2 parts, FP32 with 8); 0 -> the
+// heuristic tile, no split.
 int tiled_tile(const lime_lindblad_batch* d) {
   if (d->variant == 1 || d->variant == 3 || d->variant == 4) return 32;
   if (d->variant == 2) return 64;

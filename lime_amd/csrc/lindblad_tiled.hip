@@ -834,6 +834,37 @@ __device__ __forceinline__ void border_index(int nc, int oidx, int& bi, int& bj)
   }
 }
 
+// A border workgroup of the stage (bordered layout): the last row and column of the
+// operator as complex dot products, with the same element update and records.
+template <typename R>
+__device__ __forceinline__ void stage_border(const TiledArgs& a, int b, int set, int idx, int tiles, double2* sRed) {
+  const int n = a.n, nc = a.nc;
+  const size_t nn = (size_t)n * n;
+  const bool rec = (a.stage == 3) && a.rec >= 0;
+  const int q = threadIdx.x % BG;
+  const int oidx = (idx - tiles) * BG_OUT + threadIdx.x / BG;
+  const bool own = oidx < 2 * nc + 1;
+  int bi = 0, bj = 0;
+  if (own) border_index(nc, oidx, bi, bj);
+  R cre, cim;
+  border_dot<R>(a, b, set, bi, bj, q, own, cre, cim);
+  double2 rf = make_double2(0.0, 0.0);
+  const bool lead = own && q == 0;
+  if (lead) rf = stage_elem<R>(a, b, set, (size_t)bi * n + bj, cre, cim);
+  if (rec && a.obs) {
+    const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
+    for (int e = 0; e < a.n_eops; ++e) {
+      double pr = 0.0, pi = 0.0;
+      if (lead) {
+        const double2 ev = E0[(size_t)e * nn + (size_t)bi * n + bj];
+        pr = ev.x * rf.x - ev.y * rf.y;
+        pi = ev.x * rf.y + ev.y * rf.x;
+      }
+      obs_partial(a, b, idx, pr, pi, e, sRed);
+    }
+  }
+}
+
 // RK4 stage epilogue of one tile: stage_elem on every element of the tile, then the
 // tile's share of the records.  Called by every thread of the workgroup.
 template <int TILE, typename R, int P>
@@ -897,28 +928,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
   const bool rec = (a.stage == 3) && a.rec >= 0;
 
   if (idx >= tiles) {                         // border workgroup (uniform branch)
-    const int q = threadIdx.x % BG;
-    const int oidx = (idx - tiles) * BG_OUT + threadIdx.x / BG;
-    const bool own = oidx < 2 * nc + 1;
-    int bi = 0, bj = 0;
-    if (own) border_index(nc, oidx, bi, bj);
-    R cre, cim;
-    border_dot<R>(a, b, set, bi, bj, q, own, cre, cim);
-    double2 rf = make_double2(0.0, 0.0);
-    const bool lead = own && q == 0;
-    if (lead) rf = stage_elem<R>(a, b, set, (size_t)bi * n + bj, cre, cim);
-    if (rec && a.obs) {
-      const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
-      for (int e = 0; e < a.n_eops; ++e) {
-        double pr = 0.0, pi = 0.0;
-        if (lead) {
-          const double2 ev = E0[(size_t)e * nn + (size_t)bi * n + bj];
-          pr = ev.x * rf.x - ev.y * rf.y;
-          pi = ev.x * rf.y + ev.y * rf.x;
-        }
-        obs_partial(a, b, idx, pr, pi, e, sRed);
-      }
-    }
+    stage_border<R>(a, b, set, idx, tiles, sRed);
     return;
   }
 
@@ -933,21 +943,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
   stage_tile_epilogue<TILE, R, P>(a, b, set, idx, g, cr, ci, sRed);
 }
 
-// K-split stage products (a.ks > 1, layouts without border workgroups): ks consecutive
-// workgroups per tile, part p reducing k in [p klen, (p + 1) klen) into its partial sums;
-// lindblad_stage_sum then finishes the stage.  More workgroups for small batches.
+// K-split stage products (a.ks > 1): ks consecutive workgroups per core tile, part p
+// reducing k in [p klen, (p + 1) klen) into its partial sums; lindblad_stage_sum then
+// finishes the stage (and forms the border of the bordered layout).  More workgroups for
+// small batches.
 template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves<TILE, P>))) lindblad_stage_part(TiledArgs a) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
   __shared__ __align__(16) unsigned char smem[smem_bytes<TILE, R, P>()];
-  const int n = a.n;
-  const int tiles = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int n = a.n, nc = a.nc;
+  const int tiles = tiles_per_dim(nc, TILE) * tiles_per_dim(nc, TILE);
   const int blk0 = xcd_block();
   const int part = blk0 % a.ks, blk = blk0 / a.ks;
   const int b = blk / tiles, idx = blk % tiles;
   const size_t nn = (size_t)n * n;
-  const Geo g = geometry<TILE>(n, idx, threadIdx.x);
+  const Geo g = geometry<TILE>(nc, idx, threadIdx.x);
   constexpr int BW = TILE / 32;
   V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
@@ -965,23 +976,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
       for (int v = 0; v < 4; ++v) {
         const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
         const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
-        if (i < n && j < n) Pp[(size_t)i * n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
+        if (i < nc && j < nc) Pp[(size_t)i * n + j] = CT<R>::make(cr[p][q][v], ci[p][q][v]);
       }
 }
 
 // The stage of a K-split launch: the partial sums of every part, in part order, then the
-// same epilogue as lindblad_stage_tiled (one workgroup per tile; no border workgroups).
+// same epilogue as lindblad_stage_tiled (one workgroup per tile; the border workgroups of
+// the bordered layout as in lindblad_stage_tiled).
 template <int TILE, typename R, int P>
 __global__ void __launch_bounds__(256) lindblad_stage_sum(TiledArgs a) {
   using C = typename CT<R>::C;
   using V4 = typename CT<R>::V4;
   __shared__ double2 sRed[4];
-  const int n = a.n;
-  const int tiles = tiles_per_dim(n, TILE) * tiles_per_dim(n, TILE);
+  const int n = a.n, nc = a.nc;
+  const int tiles = tiles_per_dim(nc, TILE) * tiles_per_dim(nc, TILE);
+  const int per = tiles + a.border_wgs;
   const int blk = xcd_block();
-  const int b = blk / tiles, idx = blk % tiles;
+  const int b = blk / per, idx = blk % per;
   const size_t nn = (size_t)n * n;
-  const Geo g = geometry<TILE>(n, idx, threadIdx.x);
+  if (idx >= tiles) {                         // border workgroup (uniform branch)
+    stage_border<R>(a, b, b / a.group, idx, tiles, sRed);
+    return;
+  }
+  const Geo g = geometry<TILE>(nc, idx, threadIdx.x);
   constexpr int BW = TILE / 32;
   V4 cr[BW][BW], ci[BW][BW];
 #pragma unroll
@@ -993,7 +1010,7 @@ __global__ void __launch_bounds__(256) lindblad_stage_sum(TiledArgs a) {
         const int i = g.row0 + g.wr * (TILE / 2) + p * 16 + CT<R>::row(g.lg, v);
         const int j = g.col0 + g.wc * (TILE / 2) + q * 16 + g.lr;
         R xr = 0, xi = 0;
-        if (i < n && j < n) {
+        if (i < nc && j < nc) {
           const C* Pp = static_cast<const C*>(a.part) + (size_t)b * nn + (size_t)i * n + j;
           for (int part = 0; part < a.ks; ++part) {
             const C x = Pp[(size_t)part * a.batch * nn];
@@ -1387,8 +1404,8 @@ hipError_t tiled_apply(TiledArgs a, hipStream_t stream, int tile, bool fp64) {
 template <int TILE, typename R, int P>
 static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) {
   const int per = layout(a, TILE);
-  // K-split: 32-tiles without border workgroups only
-  if (a.ks < 1 || a.border_wgs > 0 || TILE != 32) a.ks = 1;
+  // K-split: 32-tiles only
+  if (a.ks < 1 || TILE != 32) a.ks = 1;
   for (int st = 0; st < 4; ++st) {
     a.stage = st;
     a.s_cur = (st % 2 == 0) ? s0 : s1;
@@ -1400,7 +1417,8 @@ static hipError_t run_step(TiledArgs a, hipStream_t stream, void* s0, void* s1) 
     bool split = false;
     if constexpr (TILE == 32) {
       if (a.ks > 1) {
-        hipLaunchKernelGGL((lindblad_stage_part<TILE, R, P>), dim3(per * a.batch * a.ks), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((lindblad_stage_part<TILE, R, P>), dim3((per - a.border_wgs) * a.batch * a.ks), dim3(256),
+                           0, stream, a);
         hipLaunchKernelGGL((lindblad_stage_sum<TILE, R, P>), dim3(per * a.batch), dim3(256), 0, stream, a);
         split = true;
       }
@@ -1501,16 +1519,8 @@ static int stage_wgs(int n, int n_dense, int flags, int tile) {
   return layout(t, tile);
 }
 
-// K-split parts actually used (32-tiles without border workgroups only).
-static int effective_ks(int n, int n_dense, int flags, int tile, int ks) {
-  if (ks <= 1 || tile != 32) return 1;
-  TiledArgs t = {};
-  t.n = n;
-  t.n_dense = n_dense;
-  t.flags = flags;
-  layout(t, tile);
-  return t.border_wgs > 0 ? 1 : ks;
-}
+// K-split parts actually used (32-tiles only).
+static int effective_ks(int tile, int ks) { return (ks <= 1 || tile != 32) ? 1 : ks; }
 
 static WorkLayout work_layout(int batch, int n, int group, int n_dense, int flags, bool fp64, int n_eops,
                               bool rops, int tile, int ks) {
@@ -1538,7 +1548,7 @@ static WorkLayout work_layout(int batch, int n, int group, int n_dense, int flag
   w.part = w.split + (split ? align256(mats * 4 * n * w.ldp * sizeof(unsigned short)) : 0);
   // K-split partial sums: ks x batch operators in the operand precision; parts of klen
   // (a multiple of both panel widths) reduction indices
-  w.ks = effective_ks(n, n_dense, flags, tile, ks);
+  w.ks = effective_ks(tile, ks);
   w.klen = ((n + w.ks - 1) / w.ks + 31) / 32 * 32;
   w.total = w.part + (w.ks > 1 ? align256((size_t)w.ks * op) : 0);
   return w;

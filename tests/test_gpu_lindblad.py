@@ -506,12 +506,13 @@ def test_tiled_records_are_stored_and_bitwise_reproducible(n, prec, kd):
 
 @pytest.mark.parametrize("variant", [3, 4])
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-7), ("3xbf16", 2e-6), ("fp64", 1e-13)])
-@pytest.mark.parametrize("n,kd", [(40, 1), (100, 2)])
+@pytest.mark.parametrize("n,kd", [(40, 1), (100, 2), (65, 0), (129, 0)])
 def test_ksplit_stage_matches_oracle(n, kd, prec, tol, variant):
     """Launch variants 3 and 4: the stage products of every 32 x 32 tile split into 2 or 8 K
     ranges of 32-multiples (at n = 40 with 8 ranges the last six are empty), whose partial
     sums lindblad_stage_sum adds in part order before the same epilogue: records, stored
-    states and trace renormalisation against the oracle."""
+    states and trace renormalisation against the oracle.  n = 65 and 129 run the bordered
+    layout (core tiles split, the border workgroups inside lindblad_stage_sum)."""
     rng = np.random.default_rng(950 + n + kd)
     H = rand_herm(rng, n)
     c_ops = [np.sqrt(0.02) * np.diag(np.eye(n)[i]) for i in range(n)]
