@@ -119,3 +119,22 @@ def test_workspace_query(lib):
     d.obs = 256                                                              # never dereferenced
     assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == \
         3 * _a256(4 * 129 * 129 * 16) + _a256(4 * 25 * 3 * 16)
+
+
+def test_workspace_of_the_ksplit_variants(lib):
+    """Launch variants 3 and 4 (32-tiles, the stage reduction split in 2 / 8 parts) add the
+    parts' partial sums (operand precision) to the workspace; every other variant adds
+    nothing; five variants on the per-stage path."""
+    d = _lib.LindbladBatch()
+    d.batch, d.n, d.group, d.eop_group, d.n_dense = 2, 100, 1, 1, 2
+    d.flags = _lib.LIME_F_FORCE_TILED
+    op = _a256(2 * 100 * 100 * 8)
+    base = 3 * op + _a256(2 * 2 * 100 * 100 * 8)
+    assert lib.lime_lindblad_variants(ctypes.byref(d)) == 5
+    for variant, extra in ((0, 0), (1, 0), (2, 0), (3, _a256(2 * 2 * 100 * 100 * 8)),
+                           (4, _a256(8 * 2 * 100 * 100 * 8))):
+        d.variant = variant
+        assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == base + extra, variant
+    d.flags, d.variant = _lib.LIME_F_FORCE_TILED | _lib.LIME_F_PREC_FP64, 4   # complex128 partials
+    assert lib.lime_lindblad_workspace_bytes(ctypes.byref(d)) == \
+        3 * _a256(2 * 100 * 100 * 16) + _a256(2 * 2 * 100 * 100 * 16) + _a256(8 * 2 * 100 * 100 * 16)
