@@ -152,6 +152,30 @@ def test_driven_matches_oracle_and_keeps_h0():
     assert np.array_equal(H0, H0_copy)
 
 
+def test_driven_two_drives_and_lime_files(tmp_path, monkeypatch):
+    """Two drives, a start time t0 != 0, and the file output of _lindblad_driven with
+    return_result=False (lime's Lindblad_solver.evolve always asks for the Result, as here):
+    obs.dat with lime's accumulated time column (and the empty den_mat.dat lime leaves),
+    the observables and the final state against the oracle."""
+    from lime_amd import datfile
+    from lime_amd.driven import _lindblad_driven
+    rng, H0, c_ops = model(6)
+    H1, H2 = rand_herm(rng, 6), rand_herm(rng, 6)
+    f1 = lambda t: 0.4 * np.sin(2.0 * t)
+    f2 = lambda t: 0.2 * np.cos(0.7 * t) + 0.1 * t      # real: H(t) stays Hermitian
+    rho0 = rand_dm(rng, 6)
+    P = [np.diag(np.eye(6)[i]).astype(complex) for i in range(6)]
+    H = [H0, [H1, f1], [H2, f2]]
+    monkeypatch.chdir(tmp_path)
+    rho = _lindblad_driven(H, rho0, c_ops=c_ops, e_ops=P, Nt=30, dt=0.05, t0=0.3, return_result=False)
+    ref_rho, ref, _ = ol.propagate_driven(H, rho0, c_ops, 0.05, 30, P, t0=0.3)
+    times, obs = datfile.read_obs("obs.dat")
+    assert np.array_equal(times, datfile.time_column(0.05, 30, 0.3))
+    assert close(obs, ref)
+    assert close(rho, ref_rho)
+    assert (tmp_path / "den_mat.dat").read_text() == ""
+
+
 def test_eseries_solver_on_gpu():
     from lime_amd import superoperator as so
     rng, H, c_ops = model(7, n=4)
