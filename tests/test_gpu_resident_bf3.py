@@ -161,3 +161,27 @@ def test_renormalisation_records_after_scaling():
     assert np.allclose(st[-1], r, rtol=0, atol=0)
     ref, _, _ = ol.propagate(H, rho0, c_ops, dt, nt)
     assert np.max(np.abs(r - ref)) / np.max(np.abs(ref)) < 2e-6
+
+
+def test_observable_count_at_the_lds_limit():
+    """The record slots of the observables share the kernel's LDS: up to 181 observables run
+    on the resident kernel, 182 move to the per-stage kernels (a workspace, the tiled launch
+    variants); both record the same values as the oracle."""
+    rng = np.random.default_rng(131)
+    H = rand_herm(rng, N)
+    c_ops = [np.sqrt(0.02) * np.diag(np.eye(N)[i]) for i in range(N)]
+    rho0 = rand_dm(rng, N)
+    dt, nt = 0.02, 4
+    for n_e, resident in ((181, True), (182, False)):
+        E = [rand_herm(rng, N) for _ in range(n_e)]
+        prop = _prop()(H, c_ops, precision="3xbf16")
+        d = prop._descriptor(1)
+        d.n_eops = n_e
+        d.obs = 256                                  # a non-NULL obs pointer: records requested
+        work = prop._lib.lime_lindblad_workspace_bytes(ctypes.byref(d))
+        assert (work == 0) == resident, (n_e, work)
+        _, obs, _ = prop.run(rho0, dt, nt, e_ops=E, nout=2)
+        _, ref, _ = ol.propagate(H, rho0, c_ops, dt, nt, E, nout=2)
+        err = np.max(np.abs(obs[0].cpu().numpy() - ref)) / np.max(np.abs(ref))
+        print(f"{n_e} observables ({'resident' if resident else 'tiled'}): rel err {err:.3e}")
+        assert err < 1e-5
