@@ -38,6 +38,9 @@ namespace lime {
 
 namespace {
 
+// a wave role as a type: the generic lambda below is instantiated once per role
+template <int V> struct RoleK { static constexpr int value = V; };
+
 typedef unsigned short u16;
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -359,81 +362,92 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
           ip[u][v] += t.y;
         }
       }
+      // The products of one stage, instantiated once per wave role (QK 0: column outputs of
+      // tile row ti through coefficient columns; 1, 2: row outputs through coefficient rows,
+      // sharing the B fragments of tile 0 or 1): the role is a compile-time constant inside,
+      // so the 120 MFMAs and their fragment reads form one basic block that the scheduler
+      // can interleave (wave-uniform branches per k-chunk cost 1 % of the launch).
+      auto products = [&](auto qk) {
+        constexpr int QK = decltype(qk)::value;
 #pragma unroll
-      for (int k0 = 0; k0 < KP; k0 += 32) {
-        // H S: A = H rows of tile row ti (shared), B = S columns of the wave's two tiles
-        {
-          const u16x8 hxh = frag(sH, aoff + k0), hxl = frag(sH + HPLANE, aoff + k0);
-          const u16x8 hyh = frag(sH + 2 * HPLANE, aoff + k0), hyl = frag(sH + 3 * HPLANE, aoff + k0);
-          u16x8 sxh[2], sxl[2], syh[2], syl[2];
+        for (int k0 = 0; k0 < KP; k0 += 32) {
+          // H S: A = H rows of tile row ti (shared), B = S columns of the wave's two tiles
+          {
+            const u16x8 hxh = frag(sH, aoff + k0), hxl = frag(sH + HPLANE, aoff + k0);
+            const u16x8 hyh = frag(sH + 2 * HPLANE, aoff + k0), hyl = frag(sH + 3 * HPLANE, aoff + k0);
+            u16x8 sxh[2], sxl[2], syh[2], syl[2];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int o = bSoff + 16 * u * SP + k0;
-            sxh[u] = frag(sC, o);
-            sxl[u] = frag(sC + SPLANE, o);
-            syh[u] = frag(sC + 2 * SPLANE, o);
-            syl[u] = frag(sC + 3 * SPLANE, o);
-          }
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            rp[u] = mfma3(hxh, hxl, sxh[u], sxl[u], rp[u]);   // Hx Sx
-            ip[u] = mfma3(hxh, hxl, syh[u], syl[u], ip[u]);   // Hx Sy
-            rn[u] = mfma3(hyh, hyl, syh[u], syl[u], rn[u]);   // Hy Sy
-            ip[u] = mfma3(hyh, hyl, sxh[u], sxl[u], ip[u]);   // Hy Sx
-          }
-          if (rowsk) {
-            const u16x8 k0h = frag(sK, koff + k0), k0l = frag(sK, koff + 2 * KPT + k0);
-            const u16x8 k1h = frag(sK, koff + 4 * KPT + k0), k1l = frag(sK, koff + 6 * KPT + k0);
-            if (usk == 0) {
-              q = mfma3(k0h, k0l, sxh[0], sxl[0], q);
-              q = mfma3(k1h, k1l, syh[0], syl[0], q);
-            } else {
-              q = mfma3(k0h, k0l, sxh[1], sxl[1], q);
-              q = mfma3(k1h, k1l, syh[1], syl[1], q);
+            for (int u = 0; u < 2; ++u) {
+              const int o = bSoff + 16 * u * SP + k0;
+              sxh[u] = frag(sC, o);
+              sxl[u] = frag(sC + SPLANE, o);
+              syh[u] = frag(sC + 2 * SPLANE, o);
+              syl[u] = frag(sC + 3 * SPLANE, o);
             }
-          } else {
-            q = mfma3(hxh, hxl, frag(sK, koff + 16 * KPT + k0), frag(sK, koff + 18 * KPT + k0), q);
-            q = mfma3(hyh, hyl, frag(sK, koff + 20 * KPT + k0), frag(sK, koff + 22 * KPT + k0), q);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              rp[u] = mfma3(hxh, hxl, sxh[u], sxl[u], rp[u]);   // Hx Sx
+              ip[u] = mfma3(hxh, hxl, syh[u], syl[u], ip[u]);   // Hx Sy
+              rn[u] = mfma3(hyh, hyl, syh[u], syl[u], rn[u]);   // Hy Sy
+              ip[u] = mfma3(hyh, hyl, sxh[u], sxl[u], ip[u]);   // Hy Sx
+            }
+            if constexpr (QK != 0) {
+              const u16x8 k0h = frag(sK, koff + k0), k0l = frag(sK, koff + 2 * KPT + k0);
+              const u16x8 k1h = frag(sK, koff + 4 * KPT + k0), k1l = frag(sK, koff + 6 * KPT + k0);
+              if constexpr (QK == 1) {
+                q = mfma3(k0h, k0l, sxh[0], sxl[0], q);
+                q = mfma3(k1h, k1l, syh[0], syl[0], q);
+              } else {
+                q = mfma3(k0h, k0l, sxh[1], sxl[1], q);
+                q = mfma3(k1h, k1l, syh[1], syl[1], q);
+              }
+            } else {
+              q = mfma3(hxh, hxl, frag(sK, koff + 16 * KPT + k0), frag(sK, koff + 18 * KPT + k0), q);
+              q = mfma3(hyh, hyl, frag(sK, koff + 20 * KPT + k0), frag(sK, koff + 22 * KPT + k0), q);
+            }
+          }
+          // S H^dag: A = S rows of tile row ti (shared; transposed reads of the column planes),
+          // B[k][j] = conj(H[j][k]) from H rows of the wave's tile columns
+          {
+            const int to = troff + k0 * SP;
+            const u16x8 zxh = frag_rows(sC, to), zxl = frag_rows(sC + SPLANE, to);
+            const u16x8 zyh = frag_rows(sC + 2 * SPLANE, to), zyl = frag_rows(sC + 3 * SPLANE, to);
+            u16x8 gxh[2], gxl[2], gyh[2], gyl[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const int o = bHoff + 16 * u * HP + k0;
+              gxh[u] = frag(sH, o);
+              gxl[u] = frag(sH + HPLANE, o);
+              gyh[u] = frag(sH + 2 * HPLANE, o);
+              gyl[u] = frag(sH + 3 * HPLANE, o);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              rn[u] = mfma3(zxh, zxl, gxh[u], gxl[u], rn[u]);   // Sx Hx'
+              ip[u] = mfma3(zxh, zxl, gyh[u], gyl[u], ip[u]);   // Sx Hy'
+              rn[u] = mfma3(zyh, zyl, gyh[u], gyl[u], rn[u]);   // Sy Hy'
+              in[u] = mfma3(zyh, zyl, gxh[u], gxl[u], in[u]);   // Sy Hx'
+            }
+            if constexpr (QK != 0) {
+              const u16x8 k2h = frag(sK, koff + 8 * KPT + k0), k2l = frag(sK, koff + 10 * KPT + k0);
+              const u16x8 k3h = frag(sK, koff + 12 * KPT + k0), k3l = frag(sK, koff + 14 * KPT + k0);
+              if constexpr (QK == 1) {
+                q = mfma3(k2h, k2l, gxh[0], gxl[0], q);
+                q = mfma3(k3h, k3l, gyh[0], gyl[0], q);
+              } else {
+                q = mfma3(k2h, k2l, gxh[1], gxl[1], q);
+                q = mfma3(k3h, k3l, gyh[1], gyl[1], q);
+              }
+            } else {
+              q = mfma3(zxh, zxl, frag(sK, koff + 24 * KPT + k0), frag(sK, koff + 26 * KPT + k0), q);
+              q = mfma3(zyh, zyl, frag(sK, koff + 28 * KPT + k0), frag(sK, koff + 30 * KPT + k0), q);
+            }
           }
         }
-        // S H^dag: A = S rows of tile row ti (shared; transposed reads of the column planes),
-        // B[k][j] = conj(H[j][k]) from H rows of the wave's tile columns
-        {
-          const int to = troff + k0 * SP;
-          const u16x8 zxh = frag_rows(sC, to), zxl = frag_rows(sC + SPLANE, to);
-          const u16x8 zyh = frag_rows(sC + 2 * SPLANE, to), zyl = frag_rows(sC + 3 * SPLANE, to);
-          u16x8 gxh[2], gxl[2], gyh[2], gyl[2];
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int o = bHoff + 16 * u * HP + k0;
-            gxh[u] = frag(sH, o);
-            gxl[u] = frag(sH + HPLANE, o);
-            gyh[u] = frag(sH + 2 * HPLANE, o);
-            gyl[u] = frag(sH + 3 * HPLANE, o);
-          }
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            rn[u] = mfma3(zxh, zxl, gxh[u], gxl[u], rn[u]);   // Sx Hx'
-            ip[u] = mfma3(zxh, zxl, gyh[u], gyl[u], ip[u]);   // Sx Hy'
-            rn[u] = mfma3(zyh, zyl, gyh[u], gyl[u], rn[u]);   // Sy Hy'
-            in[u] = mfma3(zyh, zyl, gxh[u], gxl[u], in[u]);   // Sy Hx'
-          }
-          if (rowsk) {
-            const u16x8 k2h = frag(sK, koff + 8 * KPT + k0), k2l = frag(sK, koff + 10 * KPT + k0);
-            const u16x8 k3h = frag(sK, koff + 12 * KPT + k0), k3l = frag(sK, koff + 14 * KPT + k0);
-            if (usk == 0) {
-              q = mfma3(k2h, k2l, gxh[0], gxl[0], q);
-              q = mfma3(k3h, k3l, gyh[0], gyl[0], q);
-            } else {
-              q = mfma3(k2h, k2l, gxh[1], gxl[1], q);
-              q = mfma3(k3h, k3l, gyh[1], gyl[1], q);
-            }
-          } else {
-            q = mfma3(zxh, zxl, frag(sK, koff + 24 * KPT + k0), frag(sK, koff + 26 * KPT + k0), q);
-            q = mfma3(zyh, zyl, frag(sK, koff + 28 * KPT + k0), frag(sK, koff + 30 * KPT + k0), q);
-          }
-        }
-      }
+      };
+      if (!rowsk) products(RoleK<0>{});
+      else if (usk == 0) products(RoleK<1>{});
+      else products(RoleK<2>{});
       // C at the owned border outputs (before the border index term and D o S)
       float bre[2] = {0.f, 0.f}, bim[2] = {0.f, 0.f};
       if (rowsk) {   // lanes 0..15: Re C[NC][j] in row 0, Im in row 1
@@ -624,8 +638,9 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       const bool rec_now = record_any && (g % a.nout) == 0 && (g / a.nout - 1) < a.rec_capacity;
       if (rec_now) {
         const int rec = g / a.nout - 1;
-        int olr = lr, olg = lg;
-        asm volatile("" : "+v"(olr), "+v"(olg));
+        int olr = lr, olg = lg, ob0 = bi(0) * n + bj(0), ob1 = bi(1) * n + bj(1);
+        asm volatile("" : "+v"(olr), "+v"(olg), "+v"(ob0), "+v"(ob1));   // formed here, not held
+        const int ob[2] = {ob0, ob1};
         if (a.obs != nullptr) {
           const double2* E0 = a.eops_t + (size_t)(b / a.eop_group) * a.n_eops * nn;
           for (int e = 0; e < a.n_eops; ++e) {
@@ -645,7 +660,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
               if (u < nbo) {
-                const double2 x = E[bi(u) * n + bj(u)];
+                const double2 x = E[ob[u]];
                 const double2 r = bRho[bslot(u)];
                 pr += x.x * r.x - x.y * r.y;
                 pi += x.x * r.y + x.y * r.x;
@@ -665,7 +680,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             for (int v = 0; v < 4; ++v) Sg[(I0 + v) * n + J0 + 16 * u + 4 * olg * n + olr] = rho[u][v];
 #pragma unroll
           for (int u = 0; u < 2; ++u)
-            if (u < nbo) Sg[bi(u) * n + bj(u)] = bRho[bslot(u)];
+            if (u < nbo) Sg[ob[u]] = bRho[bslot(u)];
         }
       }
       __syncthreads();   // S of the next step (and the record partials) are complete
@@ -690,10 +705,12 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
   // ---- write the state back
   {
     double2* R = a.rho + (size_t)b * nn;
+    int wlr = lr, wlg = lg;   // opaque: the addresses are formed here, not held from the start
+    asm volatile("" : "+v"(wlr), "+v"(wlg));
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) R[(I0 + 4 * lg + v) * n + J0 + 16 * u + lr] = rho[u][v];
+      for (int v = 0; v < 4; ++v) R[(I0 + 4 * wlg + v) * n + J0 + 16 * u + wlr] = rho[u][v];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
       if (u < nbo) R[bi(u) * n + bj(u)] = bRho[bslot(u)];
