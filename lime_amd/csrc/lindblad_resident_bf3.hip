@@ -275,13 +275,23 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
 
   // ---- border outputs owned by this lane (row outputs: lanes 0..15; column outputs: lanes
   // with lr < 2, rows 4 lg + 2 lr + u of tile row ti; the corner)
-  const bool own_row = rowsk && lane < 16;
-  const bool own_col = !rowsk && lr < 2;
-  const bool own_corner = w == L::CORNER_WAVE && lane == L::CORNER_LANE;
-  const int nbo = own_col ? 2 : (own_row || own_corner) ? 1 : 0;
-  auto bi = [&](int u) { return own_col ? I0 + 4 * lg + 2 * lr + u : NC; };
-  auto bj = [&](int u) { return own_row ? I0 + lane : NC; };   // row outputs: columns of tile column ti
-  auto bslot = [&](int u) { return own_row ? I0 + lane : own_col ? NC + I0 + 4 * lg + 2 * lr + u : 2 * NC; };
+  // (of lane ln: output u's row bi, column bj and state slot bslot; nbo outputs)
+  auto owns_row = [&](int ln) { return rowsk && ln < 16; };
+  auto owns_col = [&](int ln) { return !rowsk && (ln & 15) < 2; };
+  auto owns_corner = [&](int ln) { return w == L::CORNER_WAVE && ln == L::CORNER_LANE; };
+  auto nbo_of = [&](int ln) { return owns_col(ln) ? 2 : (owns_row(ln) || owns_corner(ln)) ? 1 : 0; };
+  auto bi_of = [&](int ln, int u) { return owns_col(ln) ? I0 + 4 * (ln >> 4) + 2 * (ln & 15) + u : NC; };
+  auto bj_of = [&](int ln) { return owns_row(ln) ? I0 + ln : NC; };   // row outputs: columns of tile column ti
+  auto bslot_of = [&](int ln, int u) {
+    return owns_row(ln) ? I0 + ln : owns_col(ln) ? NC + I0 + 4 * (ln >> 4) + 2 * (ln & 15) + u : 2 * NC;
+  };
+  const bool own_row = owns_row(lane);
+  const bool own_col = owns_col(lane);
+  const bool own_corner = owns_corner(lane);
+  const int nbo = nbo_of(lane);
+  auto bi = [&](int u) { return bi_of(lane, u); };
+  auto bj = [&](int u) { return bj_of(lane); };
+  auto bslot = [&](int u) { return bslot_of(lane, u); };
 
   // FP64 state of the owned core elements (I0 + 4 lg + v, J0 + 16 u + lr), the FP32 RK4
   // increment and D there
@@ -366,9 +376,12 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       // tile row ti through coefficient columns; 1, 2: row outputs through coefficient rows,
       // sharing the B fragments of tile 0 or 1): the role is a compile-time constant inside,
       // so the 120 MFMAs and their fragment reads form one basic block that the scheduler
-      // can interleave (wave-uniform branches per k-chunk cost 1 % of the launch).
+      // can interleave (wave-uniform branches per k-chunk cost 1 % of the launch).  The
+      // products run at raised wave priority: of the two waves on a SIMD, one still issuing
+      // MFMAs wins issue over one already in its epilogue (0.6 % of the launch).
       auto products = [&](auto qk) {
         constexpr int QK = decltype(qk)::value;
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int k0 = 0; k0 < KP; k0 += 32) {
           // H S: A = H rows of tile row ti (shared), B = S columns of the wave's two tiles
@@ -444,6 +457,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             }
           }
         }
+        __builtin_amdgcn_s_setprio(0);
       };
       if (!rowsk) products(RoleK<0>{});
       else if (usk == 0) products(RoleK<1>{});
@@ -497,7 +511,6 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
       // (no barrier: the next stage operand goes to the other buffer)
 
       const float cacc = (float)((st == 0 || st == 3) ? dt / 6.0 : dt / 3.0);
-      const float keep = (st == 0) ? 0.f : 1.f;   // the increment restarts at stage 0
       float2 sv[2][4], svb[2];
       if (st < 3) {
         // next stage operand S = rho + c k, formed in FP32 (it is split into BF16 pairs, 2^-17
@@ -509,8 +522,8 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const float kr = ip[u][v] - in[u][v], ki = -(rp[u][v] - rn[u][v]);
-            inc[u][v].x = fmaf(cacc, kr, keep * inc[u][v].x);
-            inc[u][v].y = fmaf(cacc, ki, keep * inc[u][v].y);
+            inc[u][v].x = fmaf(cacc, kr, inc[u][v].x);
+            inc[u][v].y = fmaf(cacc, ki, inc[u][v].y);
             sv[u][v] = make_float2(fmaf(cnf, kr, (float)rho[u][v].x), fmaf(cnf, ki, (float)rho[u][v].y));
           }
 #pragma unroll
@@ -520,8 +533,8 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             const int x = bslot(u);
             const double2 r = bRho[x];
             float2 ib = bInc[x];
-            ib.x = fmaf(cacc, kb[u].x, keep * ib.x);
-            ib.y = fmaf(cacc, kb[u].y, keep * ib.y);
+            ib.x = fmaf(cacc, kb[u].x, ib.x);
+            ib.y = fmaf(cacc, kb[u].y, ib.y);
             bInc[x] = ib;
             svb[u] = make_float2((float)(r.x + cnext * (double)kb[u].x), (float)(r.y + cnext * (double)kb[u].y));
           }
@@ -535,6 +548,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             const float kr = ip[u][v] - in[u][v], ki = -(rp[u][v] - rn[u][v]);
             rho[u][v].x += (double)fmaf(cacc, kr, inc[u][v].x);
             rho[u][v].y += (double)fmaf(cacc, ki, inc[u][v].y);
+            inc[u][v] = make_float2(0.f, 0.f);   // the next step's increment starts from zero
           }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
@@ -545,6 +559,7 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
             r.x += (double)fmaf(cacc, kb[u].x, ib.x);
             r.y += (double)fmaf(cacc, kb[u].y, ib.y);
             bRho[x] = r;
+            bInc[x] = make_float2(0.f, 0.f);
           }
         }
         if (renorm) {
@@ -711,9 +726,11 @@ lindblad_rk4_resident_bf3(ResidentArgs a) {
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int v = 0; v < 4; ++v) R[(I0 + 4 * wlg + v) * n + J0 + 16 * u + wlr] = rho[u][v];
+    int wl = lane;   // the same for the border outputs
+    asm volatile("" : "+v"(wl));
 #pragma unroll
     for (int u = 0; u < 2; ++u)
-      if (u < nbo) R[bi(u) * n + bj(u)] = bRho[bslot(u)];
+      if (u < nbo_of(wl)) R[bi_of(wl, u) * n + bj_of(wl)] = bRho[bslot_of(wl, u)];
   }
 }
 
