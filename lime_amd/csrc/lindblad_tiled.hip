@@ -432,7 +432,8 @@ __device__ __forceinline__ void gemm_term_bf3(const void* __restrict__ A, const 
       if (k0 + KB < ke) load_panel(k0 + KB);  // prefetch the next panel into registers
     }
     // fragment reads and MFMAs at raised wave priority: a wave in its products wins issue
-    // over waves staging the next panel (dense512: +2.5 %; the FP32 and FP64 terms: no gain)
+    // over waves staging the next panel (dense512: +2.5 %; the FP32 terms measured 0.3 %
+    // slower with it)
     __builtin_amdgcn_s_setprio(1);
     u16x8 a0h[BW], a0l[BW], a1h[BW], a1l[BW], b0h[BW], b0l[BW], b1h[BW], b1l[BW];
 #pragma unroll
